@@ -1,0 +1,9 @@
+"""caustics_amd — MI355X-native microlensing magnification hot path.
+
+Public API mirrors fbartolic/caustics (src/caustics/__init__.py:2-23) for the
+magnification path: the root-solver operator and the lensing functions built on it.
+All compute runs in hand-written HIP kernels for gfx950 (libcaustics_hip.so).
+"""
+from .ehrlich_aberth_primitive import ehrlich_aberth, poly_roots
+
+__all__ = ["ehrlich_aberth", "poly_roots"]

@@ -1,0 +1,129 @@
+// Root-solver kernel template and per-degree launcher, instantiated once per degree
+// in roots_dNN.hip so the degrees compile in parallel.
+#pragma once
+#include "caustics_hip.h"
+#include "cf32.hpp"
+#include "solver.hpp"
+
+namespace caus {
+
+constexpr int kRootsBlock = 256;
+
+template <int DEG, int MODE, bool REFCOMPAT>
+__global__ __launch_bounds__(kRootsBlock) void k_roots(const cd* __restrict__ coeffs,
+                                                       const cd* __restrict__ roots_init,
+                                                       cd* __restrict__ roots,
+                                                       int32_t* __restrict__ stats,
+                                                       uint8_t* __restrict__ status, int64_t N,
+                                                       int itmax) {
+    const int64_t i = (int64_t)blockIdx.x * kRootsBlock + threadIdx.x;
+    if (i >= N) return;
+    cd p[DEG + 1];
+#pragma unroll
+    for (int k = 0; k <= DEG; ++k) p[k] = coeffs[i * (DEG + 1) + k];
+    cd z[DEG];
+    const bool custom = roots_init != nullptr;
+    if (custom) {
+#pragma unroll
+        for (int k = 0; k < DEG; ++k) z[k] = roots_init[i * DEG + k];
+    }
+    SolveStats st;
+    bool ok;
+    if (MODE == MODE_CF32)
+        ok = solve_cf32<DEG>(p, z, itmax, custom, st);
+    else
+        ok = solve_fp64<DEG, MODE, REFCOMPAT>(p, z, itmax, custom, st);
+#pragma unroll
+    for (int k = 0; k < DEG; ++k) roots[i * DEG + k] = z[k];
+    if (status) status[i] = ok ? 1 : 0;
+    if (stats) {
+        stats[4 * i + 0] = st.sweeps;
+        stats[4 * i + 1] = st.it64;
+        stats[4 * i + 2] = st.itcomp;
+        stats[4 * i + 3] = st.it32;
+    }
+}
+
+template <int DEG, int MODE, bool RC>
+int launch_roots(hipStream_t s, const cd* c, const cd* ri, cd* r, int32_t* st, uint8_t* ok,
+                 int64_t N, int itmax) {
+    const int64_t blocks = (N + kRootsBlock - 1) / kRootsBlock;
+    hipLaunchKernelGGL((k_roots<DEG, MODE, RC>), dim3((unsigned)blocks), dim3(kRootsBlock), 0, s,
+                       c, ri, r, st, ok, N, itmax);
+    return hipGetLastError() == hipSuccess ? CAUS_OK : CAUS_ERR_LAUNCH;
+}
+
+
+// Adjoint of the root map (implicit function theorem, ehrlich_aberth_primitive.py:
+// 305-319): dz_j/dp_k = -z_j^k / p'(z_j).  For cotangents g_j the coefficient gradient
+// in the conjugate-Wirtinger convention is  gc_k = sum_j g_j conj(dz_j/dp_k).
+template <int DEG>
+__global__ __launch_bounds__(kRootsBlock) void k_roots_vjp(const cd* __restrict__ coeffs,
+                                                           const cd* __restrict__ roots,
+                                                           const cd* __restrict__ g,
+                                                           cd* __restrict__ gc, int64_t N) {
+    const int64_t i = (int64_t)blockIdx.x * kRootsBlock + threadIdx.x;
+    if (i >= N) return;
+    cd p[DEG + 1], acc[DEG + 1];
+#pragma unroll
+    for (int k = 0; k <= DEG; ++k) {
+        p[k] = coeffs[i * (DEG + 1) + k];
+        acc[k] = cd{0.0, 0.0};
+    }
+#pragma unroll
+    for (int j = 0; j < DEG; ++j) {
+        const cd zj = roots[i * DEG + j];
+        const cd gj = g[i * DEG + j];
+        cd d = (double)DEG * p[DEG];
+#pragma unroll
+        for (int k = DEG - 1; k >= 1; --k) d = cfma(d, zj, (double)k * p[k]);
+        const cd u = -(gj * conj(crcp<2>(d)));
+        const cd zc = conj(zj);
+        cd pw = cd{1.0, 0.0};
+#pragma unroll
+        for (int k = 0; k <= DEG; ++k) {
+            acc[k] = cfma(u, pw, acc[k]);
+            pw = pw * zc;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k <= DEG; ++k) gc[i * (DEG + 1) + k] = acc[k];
+}
+
+template <int DEG>
+int roots_vjp_for_degree(hipStream_t s, const cd* c, const cd* r, const cd* g, cd* gc, int64_t N) {
+    const int64_t blocks = (N + kRootsBlock - 1) / kRootsBlock;
+    hipLaunchKernelGGL((k_roots_vjp<DEG>), dim3((unsigned)blocks), dim3(kRootsBlock), 0, s, c, r, g,
+                       gc, N);
+    return hipGetLastError() == hipSuccess ? CAUS_OK : CAUS_ERR_LAUNCH;
+}
+
+// Launch for one degree; defined in roots_dNN.hip.
+template <int DEG>
+int roots_for_degree(hipStream_t s, const cd* c, const cd* ri, cd* r, int32_t* st, uint8_t* ok,
+                     int64_t N, int itmax, int mode, int init_mode);
+
+#define CAUS_DEFINE_ROOTS_DEGREE(DEG)                                                           \
+    template <>                                                                                 \
+    int roots_for_degree<DEG>(hipStream_t s, const cd* c, const cd* ri, cd* r, int32_t* st,    \
+                              uint8_t* ok, int64_t N, int itmax, int mode, int init_mode) {     \
+        const bool rc = init_mode == CAUS_INIT_REFERENCE;                                       \
+        switch (mode) {                                                                         \
+            case CAUS_MODE_FP64:                                                                \
+                return rc ? launch_roots<DEG, MODE_FP64, true>(s, c, ri, r, st, ok, N, itmax)   \
+                          : launch_roots<DEG, MODE_FP64, false>(s, c, ri, r, st, ok, N, itmax); \
+            case CAUS_MODE_FP64_COMP:                                                           \
+                return rc ? launch_roots<DEG, MODE_FP64_COMP, true>(s, c, ri, r, st, ok, N,     \
+                                                                    itmax)                      \
+                          : launch_roots<DEG, MODE_FP64_COMP, false>(s, c, ri, r, st, ok, N,    \
+                                                                     itmax);                    \
+            case CAUS_MODE_CF32:                                                                \
+                return launch_roots<DEG, MODE_CF32, false>(s, c, ri, r, st, ok, N, itmax);      \
+            default:                                                                            \
+                return CAUS_ERR_ARG;                                                            \
+        }                                                                                       \
+    }                                                                                           \
+    template int roots_vjp_for_degree<DEG>(hipStream_t, const cd*, const cd*, const cd*, cd*,   \
+                                           int64_t);
+
+}  // namespace caus

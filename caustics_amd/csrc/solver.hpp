@@ -1,0 +1,349 @@
+// Ehrlich-Aberth all-roots solver, one polynomial per lane, for gfx950.
+//
+// Algorithm (reference, fbartolic/caustics):
+//   initial estimates ........ lib/ehrlich_aberth/init_est.h:57-102 (upper convex hull of
+//                              (i, log|p_i|), one ring of roots per hull edge)
+//   FP64 iteration ........... lib/ehrlich_aberth/ehrlich_aberth.h:60-150
+//   compensated stage ........ lib/ehrlich_aberth/ehrlich_aberth.h:153-292,
+//                              lib/ehrlich_aberth/horner.h:281-385
+// Layout: coefficients ascending (p[k] multiplies z^k), held in VGPRs; every
+// loop over the degree is unrolled at compile time so no array leaves registers.
+//
+// Two iteration forms:
+//   RefCompat  the reference's branch structure: |z|>1 uses the reversed polynomial
+//              at 1/z and squares z as exp(2 log z) (ehrlich_aberth.h:56-57); with
+//              real-axis initial estimates (init_est.h:95 as compiled) it reproduces
+//              the reference's root order.
+//   Fast       complex initial estimates r(cos t + i sin t); every root uses the
+//              forward polynomial.  The forward and reversed Aberth steps are the same
+//              rational function of z and their stopping tests |h| <= eps*b compare the
+//              same ratio (both sides scale by |z|^-n), so only rounding differs; with
+//              degree <= 12 and |z| bounded by the Fujiwara radius there is no overflow.
+#pragma once
+#include "cplx.hpp"
+
+namespace caus {
+
+constexpr double EA_EPS = 1.1102230246251565e-16;  // DBL_EPSILON/2, horner.h:21
+constexpr double ALPHA_W = 3.8284271247461900976;   // 1+2*sqrt(2), ehrlich_aberth.h:96-99
+
+enum SolveMode : int { MODE_FP64 = 0, MODE_FP64_COMP = 1, MODE_CF32 = 2 };
+
+// per-polynomial counters: [0] sweeps, [1] FP64 root-iterations (plain stage),
+// [2] compensated / double-float root-iterations, [3] FP32 root-iterations
+struct SolveStats {
+    int sweeps, it64, itcomp, it32;
+};
+
+template <int N>
+__device__ __forceinline__ double sel(const double (&a)[N], int idx) {
+    double v = a[0];
+#pragma unroll
+    for (int k = 1; k < N; ++k) v = (idx == k) ? a[k] : v;
+    return v;
+}
+
+// init_est.h:57-102.  The hull is kept as a bitmask of abscissae; points are pushed
+// from x = DEG down to 0 exactly as convex_hull (init_est.h:37-54) does, so the top of
+// the stack is the lowest set bit.  Root slot m lies on the hull edge [xlo, xhi)
+// containing m, which is where the reference's running offset k places it.
+template <int DEG, bool REALAXIS>
+__device__ __forceinline__ void init_est(const double (&alpha)[DEG + 1], cd (&z)[DEG]) {
+    double y[DEG + 1];
+#pragma unroll
+    for (int i = 0; i <= DEG; ++i) y[i] = alpha[i] > 0 ? log(alpha[i]) : -1e30;
+    unsigned st = 0;
+#pragma unroll
+    for (int i = DEG; i >= 0; --i) {
+        while (__builtin_popcount(st) >= 2) {
+            int x2 = __builtin_ctz(st);
+            int x1 = __builtin_ctz(st & (st - 1));
+            double y1 = sel(y, x1), y2 = sel(y, x2);
+            double c = (double)(x2 - x1) * (y[i] - y1) - (y2 - y1) * (double)(i - x1);
+            if (c <= 0)
+                st &= st - 1;
+            else
+                break;
+        }
+        st |= 1u << i;
+    }
+    const double pi2 = 6.28318530717958647693, sigma = 0.7, th = pi2 / DEG;
+#pragma unroll
+    for (int m = 0; m < DEG; ++m) {
+        const unsigned low = (2u << m) - 1u;
+        int xlo = 31 - __builtin_clz(st & low);
+        int xhi = __builtin_ctz(st & ~low);
+        int nz = xhi - xlo;
+        int j = m - xlo;
+        int e = __builtin_popcount(st >> (xhi + 1));
+        double r = exp((sel(y, xlo) - sel(y, xhi)) / nz);
+        double t = (pi2 / nz) * j + th * e + sigma;
+        if (REALAXIS) {
+            z[m] = cd{r * sin(t), 0.0};
+        } else {
+            double s, c;
+            sincos(t, &s, &c);
+            z[m] = cd{r * c, r * s};
+        }
+    }
+}
+
+// exp(2 log z) (thrust::pow(z, 2), cpow.h:36-43)
+__device__ __forceinline__ cd pow2_explog(cd z) {
+    double lr = log(sqrt(norm2(z)));
+    double th = atan2(z.im, z.re);
+    double m = exp(2.0 * lr), s, c;
+    sincos(2.0 * th, &s, &c);
+    return cd{m * c, m * s};
+}
+
+template <int DEG>
+__device__ __forceinline__ cd aberth_sum(const cd (&z)[DEG], int j, cd x) {
+    cd S = cd{0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < DEG; ++i)
+        if (i != j) S = S + crcp<1>(x - z[i]);
+    return S;
+}
+
+// ---- EFT building blocks (horner.h:37-84, 164-217) ---------------------------
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+    s = a + b;
+    double t = s - a;
+    e = (a - (s - t)) + (b - t);
+}
+__device__ __forceinline__ void two_prod(double a, double b, double& p, double& e) {
+    p = a * b;
+    e = fma(a, b, -p);
+}
+__device__ __forceinline__ void two_prod_c(cd a, cd b, cd& r, cd& e1, cd& e2, cd& e3) {
+    double p0, q0, p1, q1, p2, q2, p3, q3, s4, r4, s5, r5;
+    two_prod(a.re, b.re, p0, q0);
+    two_prod(a.im, b.im, p1, q1);
+    two_prod(a.re, b.im, p2, q2);
+    two_prod(a.im, b.re, p3, q3);
+    two_sum(p0, -p1, s4, r4);
+    two_sum(p2, p3, s5, r5);
+    r = cd{s4, s5};
+    e1 = cd{q0, q2};
+    e2 = cd{-q1, q3};
+    e3 = cd{r4, r5};
+}
+__device__ __forceinline__ void two_sum_c(cd a, cd b, cd& s, cd& e) {
+    two_sum(a.re, b.re, s.re, e.re);
+    two_sum(a.im, b.im, s.im, e.im);
+}
+__device__ __forceinline__ void cswap_desc(double& a, double& b) {
+    bool sw = fabs(b) > fabs(a);
+    double ta = sw ? b : a, tb = sw ? a : b;
+    a = ta;
+    b = tb;
+}
+// Priest's doubly compensated sum of four terms sorted by decreasing magnitude
+// (horner.h:164-208); the sort is a 5-comparator network.
+__device__ __forceinline__ double priest4(double p0, double p1, double p2, double p3) {
+    cswap_desc(p0, p1);
+    cswap_desc(p2, p3);
+    cswap_desc(p0, p2);
+    cswap_desc(p1, p3);
+    cswap_desc(p1, p2);
+    double s = p0, c = 0, y, u, t, v, zz;
+    double q[3] = {p1, p2, p3};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        y = c + q[i];
+        u = q[i] - (y - c);
+        t = y + s;
+        v = y - (t - s);
+        zz = u + v;
+        s = t + zz;
+        c = zz - (s - t);
+    }
+    return s;
+}
+__device__ __forceinline__ double gamma_const(int n) {
+    double g = (2 * n * EA_EPS) * 1.41421356237309504880;
+    return g / ((1 - 2 * EA_EPS) - g);
+}
+
+// compensated complex Horner with running error bound (horner.h:281-385)
+template <int DEG, bool REV>
+__device__ __forceinline__ void horner_comp(const cd (&p)[DEG + 1], cd x, cd& h, cd& hd,
+                                            double& eb) {
+    cd e = cd{0, 0}, ed = cd{0, 0};
+    h = REV ? p[0] : p[DEG];
+    hd = cd{0, 0};
+    eb = 0;
+    double ax = cabs(x);
+#pragma unroll
+    for (int step = 0; step < DEG; ++step) {
+        const int i = REV ? step + 1 : DEG - 1 - step;
+        cd r, e1, e2, e3, se;
+        two_prod_c(hd, x, r, e1, e2, e3);
+        two_sum_c(r, h, hd, se);
+        ed = ed * x + e +
+             cd{priest4(e1.re, e2.re, e3.re, se.re), priest4(e1.im, e2.im, e3.im, se.im)};
+        two_prod_c(h, x, r, e1, e2, e3);
+        two_sum_c(r, p[i], h, se);
+        e = e * x + cd{priest4(e1.re, e2.re, e3.re, se.re), priest4(e1.im, e2.im, e3.im, se.im)};
+        eb = eb * ax + priest4(cabs(e1), cabs(e2), cabs(e3), cabs(se));
+    }
+    h = h + e;
+    hd = hd + ed;
+}
+
+// One compensated-stage step for root j (ehrlich_aberth.h:226-280).  Returns true
+// when the root is declared converged at the compensated level.
+template <int DEG, bool REFCOMPAT>
+__device__ __forceinline__ bool comp_step(const cd (&p)[DEG + 1], cd (&z)[DEG], int j) {
+    cd zj = z[j];
+    double az = cabs(zj);
+    cd h, hd;
+    double b;
+    const double g = gamma_const(4 * DEG + 2);
+    if (az > 1) {
+        cd x = crcp<2>(zj);
+        horner_comp<DEG, true>(p, x, h, hd, b);
+        double ah = cabs(h);
+        double eb = EA_EPS * ah + (g * b + 2 * EA_EPS * EA_EPS * ah);
+        if (ah > 4 * eb) {
+            cd S = aberth_sum<DEG>(z, j, zj);
+            cd z2 = REFCOMPAT ? pow2_explog(zj) : zj * zj;
+            cd num = z2 * h;
+            cd den = ((double)DEG * zj) * h - hd - num * S;
+            cd corr = cdiv<2>(num, den);
+            if (cabs(corr) > 4 * EA_EPS * az) {
+                z[j] = zj - corr;
+                return false;
+            }
+        }
+        return true;
+    } else {
+        horner_comp<DEG, false>(p, zj, h, hd, b);
+        double ah = cabs(h);
+        double eb = EA_EPS * ah + (g * b + 2 * EA_EPS * EA_EPS * ah);
+        if (ah > 4 * eb) {
+            cd S = aberth_sum<DEG>(z, j, zj);
+            cd corr = cdiv<2>(h, hd - h * S);
+            if (cabs(corr) > 4 * EA_EPS) {
+                z[j] = zj - corr;
+                return false;
+            }
+        }
+        return true;
+    }
+}
+
+// Plain FP64 step for root j, forward polynomial only (Fast form).
+template <int DEG>
+__device__ __forceinline__ bool fast_step(const cd (&p)[DEG + 1], const double (&al)[DEG + 1],
+                                          cd (&z)[DEG], int j) {
+    cd x = z[j];
+    double ax = sqrt(norm2(x));
+    double b = al[DEG];
+#pragma unroll
+    for (int k = DEG - 1; k >= 0; --k) b = fma(b, ax, al[k]);
+    cd h = p[DEG], hd = cd{0.0, 0.0};
+#pragma unroll
+    for (int k = DEG - 1; k >= 0; --k) {
+        hd = cfma(hd, x, h);
+        h = cfma(h, x, p[k]);
+    }
+    const double tb = EA_EPS * b;
+    if (norm2(h) > tb * tb) {
+        cd S = aberth_sum<DEG>(z, j, x);
+        z[j] = x - cdiv<2>(h, hd - h * S);
+        return false;
+    }
+    return true;
+}
+
+// Plain FP64 step for root j with the reference's branch structure (RefCompat form).
+template <int DEG>
+__device__ __forceinline__ bool ref_step(const cd (&p)[DEG + 1], const double (&al)[DEG + 1],
+                                         cd (&z)[DEG], int j, int& nrev) {
+    cd zj = z[j];
+    double az = cabs(zj);
+    double b;
+    cd h, hd = cd{0.0, 0.0};
+    if (az > 1) {
+        ++nrev;
+        double ax = rcp(az);
+        cd x = crcp<2>(zj);
+        b = al[0];
+#pragma unroll
+        for (int k = 1; k <= DEG; ++k) b = fma(b, ax, al[k]);
+        h = p[0];
+#pragma unroll
+        for (int k = 1; k <= DEG; ++k) {
+            hd = cfma(hd, x, h);
+            h = cfma(h, x, p[k]);
+        }
+        if (cabs(h) > EA_EPS * b) {
+            cd S = aberth_sum<DEG>(z, j, zj);
+            cd z2 = pow2_explog(zj);
+            cd num = z2 * h;
+            cd den = ((double)DEG * zj) * h - hd - num * S;
+            z[j] = zj - cdiv<2>(num, den);
+            return false;
+        }
+        return true;
+    } else {
+        b = al[DEG];
+#pragma unroll
+        for (int k = DEG - 1; k >= 0; --k) b = fma(b, az, al[k]);
+        h = p[DEG];
+#pragma unroll
+        for (int k = DEG - 1; k >= 0; --k) {
+            hd = cfma(hd, zj, h);
+            h = cfma(h, zj, p[k]);
+        }
+        if (cabs(h) > EA_EPS * b) {
+            cd S = aberth_sum<DEG>(z, j, zj);
+            z[j] = zj - cdiv<2>(h, hd - h * S);
+            return false;
+        }
+        return true;
+    }
+}
+
+// Solve one polynomial held in registers.  Returns true when all roots converged
+// (at the compensated level for MODE_FP64_COMP).  z holds the initial estimates on
+// entry when custom_init is set.
+template <int DEG, int MODE, bool REFCOMPAT>
+__device__ __forceinline__ bool solve_fp64(const cd (&p)[DEG + 1], cd (&z)[DEG], int itmax,
+                                           bool custom_init, SolveStats& st) {
+    double al[DEG + 1];
+#pragma unroll
+    for (int i = 0; i <= DEG; ++i) al[i] = cabs(p[i]);
+    if (!custom_init) init_est<DEG, REFCOMPAT>(al, z);
+#pragma unroll
+    for (int i = 0; i <= DEG; ++i) al[i] = al[i] * fma(ALPHA_W, (double)i, 1.0);
+    constexpr unsigned ALL = (1u << DEG) - 1u;
+    unsigned cx = 0, cy = 0;
+    int nrev = 0;
+    st.sweeps = 0;
+    st.it64 = 0;
+    st.itcomp = 0;
+    st.it32 = 0;
+    bool done = false;
+    for (int it = 0; it < itmax; ++it) {
+#pragma unroll
+        for (int j = 0; j < DEG; ++j) {
+            if (!((cx >> j) & 1u)) {
+                ++st.it64;
+                bool c = REFCOMPAT ? ref_step<DEG>(p, al, z, j, nrev) : fast_step<DEG>(p, al, z, j);
+                if (c) cx |= 1u << j;
+            } else if (MODE == MODE_FP64_COMP && !((cy >> j) & 1u)) {
+                ++st.itcomp;
+                if (comp_step<DEG, REFCOMPAT>(p, z, j)) cy |= 1u << j;
+            }
+        }
+        st.sweeps = it + 1;
+        done = (MODE == MODE_FP64_COMP) ? (cy == ALL) : (cx == ALL);
+        if (done) break;
+    }
+    return done;
+}
+
+}  // namespace caus

@@ -1,0 +1,49 @@
+"""Shared checks for the parity tests."""
+import numpy as np
+from scipy.optimize import linear_sum_assignment
+
+EPS = 2.0 ** -53
+
+
+def match_rows(a, b):
+    """Permute each row of b to best match the same row of a (assignment on |a_i-b_j|)."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    out = b.copy()
+    same = np.all(np.abs(a - b) <= 1e-8 * (1 + np.abs(a)), axis=1)
+    for i in np.nonzero(~same)[0]:
+        d = np.abs(a[i][:, None] - b[i][None, :])
+        r, c = linear_sum_assignment(d)
+        out[i, r] = b[i, c]
+    return out
+
+
+def root_condition(coeffs_asc, z):
+    """Relative condition number of each root: sum_k |p_k||z|^k / (|z| |p'(z)|)."""
+    c = np.asarray(coeffs_asc)
+    n = c.shape[-1] - 1
+    k = np.arange(n + 1)
+    az = np.abs(z)[..., None]
+    num = (np.abs(c)[:, None, :] * az ** k).sum(-1)
+    dp = (k[1:] * c[:, None, 1:] * z[..., None] ** (k[1:] - 1)).sum(-1)
+    return num / np.maximum(np.abs(z) * np.abs(dp), 1e-300)
+
+
+def residual_ratio(coeffs_asc, z):
+    """|p(z)| / sum_k |p_k||z|^k (normwise backward error of each root)."""
+    c = np.asarray(coeffs_asc)
+    n = c.shape[-1] - 1
+    k = np.arange(n + 1)
+    pz = (c[:, None, :] * z[..., None] ** k).sum(-1)
+    den = (np.abs(c)[:, None, :] * np.abs(z)[..., None] ** k).sum(-1)
+    return np.abs(pz) / den
+
+
+def roots_close(z_test, z_ref, coeffs_asc, rel=1e-12, kappa_factor=64.0):
+    """Matched root sets agree to `rel` relative, or within kappa_factor*kappa*eps
+    for ill-conditioned roots.  Returns (ok mask, matched z_test, rel error)."""
+    zt = match_rows(z_ref, z_test)
+    err = np.abs(zt - z_ref) / np.maximum(np.abs(z_ref), 1e-300)
+    kap = root_condition(coeffs_asc, z_ref)
+    tol = np.maximum(rel, kappa_factor * kap * EPS)
+    return err <= tol, zt, err

@@ -1,0 +1,148 @@
+"""GPU root solver vs the C oracle (oracle/ea_oracle.c) through the C ABI.
+
+Bar: matched root sets within 1e-12 relative, or 64*kappa*eps for ill-conditioned
+roots (kappa = sum|p_k||z|^k/(|z||p'(z)|)); with the reference's initial estimates
+the root ORDER equals the oracle's for (almost) every polynomial."""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+import caustics_amd as ca
+from caustics_amd import _lib
+from caustics_amd import workloads as W
+from caustics_amd.ehrlich_aberth_primitive import solve_with_stats
+from oracle import ea, lens
+
+from helpers import residual_ratio, roots_close
+
+pytestmark = pytest.mark.gpu
+
+
+def lens_polys(nlenses, n, seed):
+    lf, w, _ = W.batched_polys(nlenses, n, seed)
+    return W.coeffs_product(w, lf["a"], lf["e1"], lf["e2"], lf["r3"], nlenses)
+
+
+@pytest.mark.parametrize("nlenses,n", [(2, 20000), (3, 6000)])
+@pytest.mark.parametrize("mode", ["fp64", "fp64_comp", "cf32"])
+@pytest.mark.parametrize("init", ["reference", "complex"])
+def test_lens_roots_match_oracle(gpu, nlenses, n, mode, init):
+    if mode == "cf32" and init == "reference":
+        pytest.skip("cf32 always uses complex initial estimates")
+    c = lens_polys(nlenses, n, seed=100 + nlenses)
+    z_or, _, st_or = ea.roots_asc(c, compensated=(mode == "fp64_comp"),
+                                   init_mode=0 if init == "reference" else 1, return_stats=True)
+    z, status, stats = solve_with_stats(c, itmax=2000, mode=mode, init=init)
+    z = z.cpu().numpy()
+    status = status.cpu().numpy()
+    print(f"[{nlenses=} {mode=} {init=}] unconverged gpu={int((status == 0).sum())} "
+          f"oracle={int((st_or == 0).sum())}")
+    assert status.mean() > 0.999
+    ok, zm, err = roots_close(z, z_or, c)
+    good = status.astype(bool) & st_or.astype(bool)
+    assert ok[good].all(), (err[good][~ok[good]][:10])
+    assert np.max(residual_ratio(c, z)[good]) < 1e-13
+    if init == "reference" and mode == "fp64":
+        same_order = np.all(zm == z, axis=1)
+        assert same_order.mean() > 0.99
+
+
+def test_reference_fixture_real_coefficients(gpu):
+    # tests/test_ehrlich_aberth_primitive.py:19-35: real source positions give real
+    # coefficients; the reference init escapes the real axis through exp(2 log z)
+    w = np.linspace(0.3, 0.35, 10).astype(complex)
+    c = lens.poly_coeffs_binary(w, 0.45, 0.8).reshape(5, 2, 6)
+    for comp in (False, True):
+        r = ca.poly_roots(c, compensated=comp).cpu().numpy()
+        assert r.shape == (5, 2, 5)
+        for idx in [(3, 1), (4, 0)]:
+            np.testing.assert_allclose(np.polyval(c[idx], r[idx]), 0, atol=1e-10)
+        ro = ea.poly_roots(c, compensated=comp)
+        np.testing.assert_allclose(r, ro, rtol=1e-12, atol=1e-14)
+
+
+def test_batching_shapes_equivalent(gpu):
+    # vmap over either axis in the reference reduces to the flat batch
+    w = np.linspace(0.3, 0.35, 10).astype(complex) + 0.05j
+    c = lens.poly_coeffs_binary(w, 0.45, 0.8).reshape(5, 2, 6)
+    r = ca.poly_roots(c).cpu().numpy()
+    r2 = ca.poly_roots(np.swapaxes(c, 0, 1)).cpu().numpy()
+    np.testing.assert_array_equal(np.swapaxes(r2, 0, 1), r)
+    flat = ca.ehrlich_aberth(c.reshape(-1, 6)[:, ::-1].copy()).cpu().numpy()
+    np.testing.assert_array_equal(flat, r.reshape(-1))
+
+
+@pytest.mark.parametrize("deg", list(range(1, 13)))
+def test_all_degrees_known_roots(gpu, deg):
+    g = np.random.default_rng(deg)
+    zs = g.normal(size=(2000, deg)) + 1j * g.normal(size=(2000, deg))
+    c = np.array([np.poly(z) for z in zs])[:, ::-1].copy()
+    for init in ("reference", "complex"):
+        z, status, _ = solve_with_stats(c, mode="fp64", init=init)
+        assert status.cpu().numpy().all()
+        ok, _, err = roots_close(z.cpu().numpy(), zs, c, rel=1e-10)
+        assert ok.all()
+
+
+def test_custom_init(gpu):
+    c = lens_polys(2, 4000, seed=5)
+    z0 = ea.roots_asc(c)
+    g = np.random.default_rng(0)
+    init = z0 + 1e-4 * (g.normal(size=z0.shape) + 1j * g.normal(size=z0.shape))
+    zo = ea.roots_asc(c, roots_init=init)
+    zg = ca.ehrlich_aberth(c, roots_init=init, custom_init=True).cpu().numpy().reshape(zo.shape)
+    ok, zm, _ = roots_close(zg, zo, c)
+    assert ok.all()
+    assert np.mean(np.all(zm == zg, axis=1)) > 0.99  # warm start keeps the order
+
+
+def test_non_convergence_reported(gpu):
+    c = lens_polys(2, 1000, seed=9)
+    _, status, stats = solve_with_stats(c, itmax=1, mode="fp64", init="complex")
+    assert not status.cpu().numpy().any()
+    assert (stats.cpu().numpy()[:, 0] == 1).all()
+
+
+def test_descriptor_entry_point(gpu):
+    c = lens_polys(2, 1000, seed=13)
+    ct = torch.as_tensor(c, device="cuda")
+    out = torch.zeros((1000, 5), dtype=torch.complex128, device="cuda")
+    desc = struct_bytes = (ctypes.c_int * 3)(1000, 5, 2000)
+    raw = bytes(desc) + bytes([0]) + bytes(3)  # int size, deg, itmax; bool compensated; pad
+    bufs = (ctypes.c_void_p * 2)(ct.data_ptr(), out.data_ptr())
+    rc = _lib.lib().caus_gpu_ehrlich_aberth(_lib.stream_ptr(), bufs, raw, len(raw))
+    assert rc == 0
+    z = ca.ehrlich_aberth(c).cpu().numpy().reshape(1000, 5)
+    np.testing.assert_array_equal(out.cpu().numpy(), z)
+
+
+def test_roots_vjp_matches_implicit_derivative_and_fd(gpu):
+    c = lens_polys(2, 64, seed=21)
+    ct = torch.as_tensor(c, device="cuda").requires_grad_(True)
+    z = ca.ehrlich_aberth(ct).reshape(64, 5)
+    g = np.random.default_rng(1)
+    v = g.normal(size=(64, 5)) + 1j * g.normal(size=(64, 5))
+    loss = (torch.as_tensor(v, device="cuda") * z).real.sum()
+    loss.backward()
+    grad = ct.grad.cpu().numpy()
+    # closed form: dL/dRe(p_k) + i dL/dIm(p_k) with dz/dp_k = -z^k/p'(z)
+    zn = z.detach().cpu().numpy()
+    k = np.arange(6)
+    dp = (k[1:] * c[:, None, 1:] * zn[..., None] ** (k[1:] - 1)).sum(-1)
+    J = -(zn[..., None] ** k) / dp[..., None]           # (N, 5, 6)
+    gre = np.real(v[..., None] * J).sum(1)
+    gim = np.real(v[..., None] * 1j * J).sum(1)
+    np.testing.assert_allclose(grad, gre + 1j * gim, rtol=1e-9, atol=1e-12)
+    # central finite differences through the oracle solver (h = 1e-6)
+    h = 1e-6
+    for kk in (0, 3, 5):
+        cp, cm = c.copy(), c.copy()
+        cp[:, kk] += h
+        cm[:, kk] -= h
+        zp, zm_ = ea.roots_asc(cp), ea.roots_asc(cm)
+        from helpers import match_rows
+        zp, zm_ = match_rows(zn, zp), match_rows(zn, zm_)
+        fd = np.real(v * (zp - zm_)).sum(1) / (2 * h)
+        np.testing.assert_allclose(grad[:, kk].real, fd, rtol=1e-4, atol=1e-6)

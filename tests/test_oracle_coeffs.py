@@ -1,0 +1,58 @@
+"""Pins the coefficient restatements (oracle/lens.py, caustics_amd/workloads.py).
+
+Golden: tests/golden/coeffs_fixture.npz holds the reference notebook's printed
+sympy polynomials (notebooks/ComplexPolynomialCoefficients.ipynb cells 10, 13, 19, 22)
+evaluated at 50 digits.  Invariant: images z mapped forward by the lens equation give
+w with p(z; w) = 0."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import lens
+from caustics_amd import workloads as W
+
+FIX = os.path.join(os.path.dirname(__file__), "golden", "coeffs_fixture.npz")
+
+
+def rel(x, y):
+    return np.max(np.abs(x - y) / np.max(np.abs(y), axis=-1, keepdims=True))
+
+
+def test_binary_expanded_matches_notebook():
+    f = np.load(FIX)
+    c = np.array([lens.poly_coeffs_binary(f["w"][i], f["a"][i], f["e1"][i]) for i in range(64)])
+    assert rel(c, f["lens2"]) < 1e-14
+
+
+def test_product_form_matches_notebook_binary_and_triple():
+    f = np.load(FIX)
+    c2 = W.coeffs_product(f["w"], f["a"], f["e1"], 1 - f["e1"], 0 * f["r3"], 2)[..., ::-1]
+    c3 = W.coeffs_product(f["w"], f["a"], f["e1"], f["e2"], f["r3"], 3)[..., ::-1]
+    assert rel(c2, f["lens2"]) < 1e-14
+    assert rel(c3, f["lens3"]) < 1e-14
+    # the oracle's own triple builder agrees too
+    c3o = np.array([lens.poly_coeffs_triple(f["w"][i], f["a"][i], f["r3"][i], f["e1"][i],
+                                            f["e2"][i]) for i in range(64)])
+    assert rel(c3o, f["lens3"]) < 1e-14
+
+
+@pytest.mark.parametrize("nlenses", [2, 3])
+def test_forward_map_invariant(nlenses):
+    g = np.random.default_rng(5)
+    n = 400
+    z = g.uniform(-2, 2, n) + 1j * g.uniform(-2, 2, n)
+    a = g.uniform(0.15, 1.5, n)
+    e1 = g.uniform(0.05, 0.9, n)
+    e2 = (1 - e1) * g.uniform(0.05, 0.95, n)
+    r3 = g.uniform(0.3, 2, n) * np.exp(1j * g.uniform(0, 2 * np.pi, n))
+    if nlenses == 2:
+        w = lens.lens_eq(z, 2, a=a, e1=e1)
+        c = W.coeffs_product(w, a, e1, 1 - e1, 0 * r3, 2)
+    else:
+        w = lens.lens_eq(z, 3, a=a, e1=e1, e2=e2, r3=r3)
+        c = W.coeffs_product(w, a, e1, e2, r3, 3)
+    k = np.arange(c.shape[-1])
+    pz = (c * z[:, None] ** k).sum(-1)
+    scale = (np.abs(c) * np.abs(z[:, None]) ** k).sum(-1)
+    assert np.max(np.abs(pz) / scale) < 1e-13
