@@ -5,5 +5,18 @@ magnification path: the root-solver operator and the lensing functions built on 
 All compute runs in hand-written HIP kernels for gfx950 (libcaustics_hip.so).
 """
 from .ehrlich_aberth_primitive import ehrlich_aberth, poly_roots
+from .extended_source import mag_extended_source
+from .lightcurve import hexadecapole_gate, mag, mag_point_source_lightcurve
+from .point_source import lens_eq, lens_eq_det_jac, mag_point_source
 
-__all__ = ["ehrlich_aberth", "poly_roots"]
+__all__ = [
+    "ehrlich_aberth",
+    "poly_roots",
+    "lens_eq",
+    "lens_eq_det_jac",
+    "mag_point_source",
+    "mag_point_source_lightcurve",
+    "mag_extended_source",
+    "hexadecapole_gate",
+    "mag",
+]

@@ -36,6 +36,22 @@ SIGNATURES = {
     "caus_gpu_ehrlich_aberth": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.c_char_p,
                                                ctypes.c_size_t]),
     "caus_roots_vjp": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i64, _i32]),
+    "caus_mag_point_source": (ctypes.c_int, [_vp, _i32, _vp, _i64, _vp, _i64, _vp, _vp, _i32,
+                                             _i32]),
+    "caus_mag_point_source_vjp": (ctypes.c_int, [_vp, _i32, _vp, _i64, _vp, _i64, _vp, _vp, _vp,
+                                                 _vp, _vp, _i32, _i32]),
+    "caus_lightcurve_workspace_bytes": (ctypes.c_int64, [_i32, _i64, _i64]),
+    "caus_lightcurve_point_source": (ctypes.c_int, [_vp, _i32, _vp, _i64, _vp, _i64, _vp, _vp,
+                                                    _vp, _vp, _i64, _vp, _i32, _i32]),
+    "caus_extended_workspace_bytes": (ctypes.c_int64, [_i32, _i64, _i32]),
+    "caus_mag_extended_source": (ctypes.c_int, [_vp, _i32, _vp, _i64, _vp, _vp, _i64, _vp, _i32,
+                                                _i32, _vp, _i64, _i32, _vp, _vp, _vp, _i64, _i32,
+                                                _i32]),
+    "caus_mag_extended_source_vjp": (ctypes.c_int, [_vp, _i32, _vp, _i64, _vp, _vp, _i64, _vp,
+                                                    _i32, _i32, _vp, _i64, _i32, _vp, _vp, _vp,
+                                                    _vp, _vp, _i64, _i32, _i32]),
+    "caus_hexadecapole_gate": (ctypes.c_int, [_vp, _i32, _vp, _i64, _vp, _i64, _vp, _i64,
+                                              ctypes.c_double, _vp, _vp, _vp, _vp, _i32]),
 }
 
 _LIB = None

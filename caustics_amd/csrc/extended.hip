@@ -1,0 +1,81 @@
+// C-ABI entry points of the extended-source contour kernels.
+#include "extended_impl.hpp"
+
+extern "C" int caus_mag_extended_source_vjp(void*, int32_t, const double*, int64_t, const double*,
+                                            const double*, int64_t, const int32_t*, int32_t,
+                                            int32_t, const double*, int64_t, int32_t,
+                                            const double*, double*, double*, uint8_t*, void*,
+                                            int64_t, int32_t, int32_t);
+
+using namespace caus;
+
+extern "C" int64_t caus_extended_workspace_bytes(int32_t nlenses, int64_t S, int32_t npts_max) {
+    if (nlenses < 1 || nlenses > 3 || S < 0 || npts_max < 4 || npts_max > 2 * kExtSortMax)
+        return -1;
+    const int deg = nlenses == 1 ? 2 : nlenses * nlenses + 1;
+    return (int64_t)ext_ws_bytes(deg, ext_mmax(npts_max)) * S;
+}
+
+extern "C" int caus_mag_extended_source(void* stream, int32_t nlenses, const double* w0, int64_t S,
+                                        const double* rho, const double* params,
+                                        int64_t params_stride, const int32_t* npts_limb,
+                                        int32_t npts_max, int32_t limb_darkening,
+                                        const double* u1, int64_t u1_stride, int32_t npts_ld,
+                                        double* mag, uint8_t* status, void* workspace,
+                                        int64_t workspace_bytes, int32_t itmax, int32_t mode) {
+    return caus_mag_extended_source_vjp(stream, nlenses, w0, S, rho, params, params_stride,
+                                        npts_limb, npts_max, limb_darkening, u1, u1_stride,
+                                        npts_ld, nullptr, mag, nullptr, status, workspace,
+                                        workspace_bytes, itmax, mode);
+}
+
+extern "C" int caus_mag_extended_source_vjp(void* stream, int32_t nlenses, const double* w0,
+                                            int64_t S, const double* rho, const double* params,
+                                            int64_t params_stride, const int32_t* npts_limb,
+                                            int32_t npts_max, int32_t limb_darkening,
+                                            const double* u1, int64_t u1_stride, int32_t npts_ld,
+                                            const double* g_mag, double* mag, double* g_params,
+                                            uint8_t* status, void* workspace,
+                                            int64_t workspace_bytes, int32_t itmax, int32_t mode) {
+    if (S < 0 || nlenses < 1 || nlenses > 3 || itmax < 0 || params_stride < 0 || u1_stride < 0)
+        return CAUS_ERR_ARG;
+    if (mode != CAUS_MODE_FP64 && mode != CAUS_MODE_CF32) return CAUS_ERR_ARG;
+    if (npts_max < 4 || ext_mmax(npts_max) > kExtSortMax) return CAUS_ERR_ARG;
+    if (limb_darkening && (npts_ld < 2 || npts_ld > 256 || !u1)) return CAUS_ERR_ARG;
+    if (S == 0) return CAUS_OK;
+    if (!w0 || !rho || !mag || !workspace || (nlenses > 1 && !params)) return CAUS_ERR_ARG;
+    if (S > 0x7fffffff) return CAUS_ERR_ARG;
+    const int deg = nlenses == 1 ? 2 : nlenses * nlenses + 1;
+    const size_t per = ext_ws_bytes(deg, ext_mmax(npts_max));
+    if (workspace_bytes < (int64_t)(per * (size_t)S)) return CAUS_ERR_ARG;
+    ExtArgs a;
+    a.w0 = reinterpret_cast<const cd*>(w0);
+    a.S = S;
+    a.rho = rho;
+    a.params = params;
+    a.pstride = params_stride;
+    a.npts = npts_limb;
+    a.npts_def = npts_max;
+    a.u1 = u1;
+    a.u1stride = u1_stride;
+    a.npts_ld = npts_ld;
+    a.mag = mag;
+    a.status = status;
+    a.g_mag = g_mag;
+    a.g_out = g_params;
+    const bool vjp = g_mag != nullptr;
+    if (vjp && !g_params) return CAUS_ERR_ARG;
+    a.ws = reinterpret_cast<char*>(workspace);
+    a.ws_per_source = per;
+    a.mmax = ext_mmax(npts_max);
+    a.itmax = itmax;
+    hipStream_t s = (hipStream_t)stream;
+    switch (nlenses) {
+        case 1:
+            return mag_extended_nl<1>(s, a, mode, limb_darkening != 0, vjp);
+        case 2:
+            return mag_extended_nl<2>(s, a, mode, limb_darkening != 0, vjp);
+        default:
+            return mag_extended_nl<3>(s, a, mode, limb_darkening != 0, vjp);
+    }
+}

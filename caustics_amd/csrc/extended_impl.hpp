@@ -1,0 +1,1310 @@
+// Extended-source magnification by contour integration: one workgroup per source.
+//
+// Reference (fbartolic/caustics, src/caustics/):
+//   limb sampling + 10 gap-bisection rounds .... extended_source.py:56-153
+//   image continuation (greedy match_points) ... extended_source.py:34-53, utils.py:15-40
+//   segments: closure test, split, ordering .... extended_source.py:156-309
+//   stitching of open segments ................. extended_source.py:312-667
+//   contours, magnification .................... extended_source.py:670-901
+//   trapezoid / limb-darkened P,Q integrals .... integrate.py:18-121, utils.py:89-100
+//
+// Layout per source (workspace): limb angles, images (point-major: z[k*DEG + row]),
+// per-point flag words (bit r: real image in row r; bit 16+r: det J < 0), all double
+// buffered for the insertions of the refinement rounds, plus one 64-bit word per point
+// for the image permutations and a P/Q scratch for limb darkening.
+//
+// Identity of images along the limb: every limb point is solved from the default
+// initial estimates in parallel (one lane per point); image rows are then chained with
+// the reference's greedy nearest-point assignment.  Where the nearest-neighbour map
+// between consecutive points is a bijection the greedy result does not depend on the
+// row order, so those steps compose as permutations; the rare steps where two images
+// compete are resolved with the greedy rule on the actual carried rows (wave 0).
+// Segments are kept as lists of pieces (row, first point, length, direction) instead
+// of dense zero-padded arrays; stitching reproduces the reference's ordering rules.
+#pragma once
+#include "caustics_hip.h"
+#include "ps_impl.hpp"
+
+namespace caus {
+
+constexpr int kExtBlock = 256;
+constexpr int kExtMaxSeg = 32;     // >= 3 * (nlenses^2 + 1)
+constexpr int kExtMaxPieces = 48;  // pieces of one stitched contour
+constexpr int kExtSortMax = 4096;  // limb points per source (gaps sorted in LDS)
+
+template <int NL>
+constexpr int ext_degree() {
+    return NL == 1 ? 2 : NL * NL + 1;
+}
+
+__host__ __device__ inline int ext_npts_init(int npts) { return npts / 2; }
+__host__ __device__ inline int ext_n_add(int npts) { return (npts / 2) / 10; }
+__host__ __device__ inline int ext_mmax(int npts) { return ext_npts_init(npts) + 10 * ext_n_add(npts); }
+
+__host__ __device__ inline size_t ext_align(size_t x) { return (x + 255) & ~size_t(255); }
+
+// bytes of workspace per source
+__host__ __device__ inline size_t ext_ws_bytes(int deg, int mmax) {
+    size_t b = 0;
+    b += ext_align(2 * (size_t)mmax * sizeof(double));            // theta x2
+    b += ext_align(2 * (size_t)mmax * deg * sizeof(cd));          // z x2
+    b += ext_align(2 * (size_t)mmax * sizeof(uint32_t));          // flags x2
+    b += ext_align((size_t)(mmax + 1) * sizeof(uint64_t));        // permutations / changepoints
+    b += ext_align(2 * ((size_t)mmax * deg + 2) * sizeof(double)); // P, Q
+    return b;
+}
+
+struct ExtWS {
+    double* theta[2];
+    cd* z[2];
+    uint32_t* flg[2];
+    uint64_t* perm;
+    double* pq;
+};
+
+__device__ inline ExtWS ext_ws(char* base, int deg, int mmax) {
+    ExtWS w;
+    size_t o = 0;
+    w.theta[0] = (double*)(base + o);
+    w.theta[1] = w.theta[0] + mmax;
+    o += ext_align(2 * (size_t)mmax * sizeof(double));
+    w.z[0] = (cd*)(base + o);
+    w.z[1] = w.z[0] + (size_t)mmax * deg;
+    o += ext_align(2 * (size_t)mmax * deg * sizeof(cd));
+    w.flg[0] = (uint32_t*)(base + o);
+    w.flg[1] = w.flg[0] + mmax;
+    o += ext_align(2 * (size_t)mmax * sizeof(uint32_t));
+    w.perm = (uint64_t*)(base + o);
+    o += ext_align((size_t)(mmax + 1) * sizeof(uint64_t));
+    w.pq = (double*)(base + o);
+    return w;
+}
+
+// ---- permutations packed as 4-bit entries ------------------------------------
+__device__ __forceinline__ int pget(uint64_t p, int i) { return (int)((p >> (4 * i)) & 0xF); }
+template <int DEG>
+__device__ __forceinline__ uint64_t pident() {
+    uint64_t p = 0;
+#pragma unroll
+    for (int i = 0; i < DEG; ++i) p |= (uint64_t)i << (4 * i);
+    return p;
+}
+// (a o b)[i] = a[b[i]]
+template <int DEG>
+__device__ __forceinline__ uint64_t pcompose(uint64_t a, uint64_t b) {
+    uint64_t r = 0;
+#pragma unroll
+    for (int i = 0; i < DEG; ++i) r |= (uint64_t)pget(a, pget(b, i)) << (4 * i);
+    return r;
+}
+constexpr uint64_t kConflictBit = 1ull << 63;
+
+// greedy match_points(a, b): for i in order, nearest unused b (ties -> lowest index)
+template <int DEG>
+__device__ __forceinline__ uint64_t greedy_match(const cd (&a)[DEG], const cd (&b)[DEG]) {
+    unsigned used = 0;
+    uint64_t p = 0;
+#pragma unroll
+    for (int i = 0; i < DEG; ++i) {
+        double best = 0.0;
+        int bj = -1;
+#pragma unroll
+        for (int j = 0; j < DEG; ++j) {
+            const double d = norm2(b[j] - a[i]);
+            if (!((used >> j) & 1u) && (bj < 0 || d < best)) {
+                best = d;
+                bj = j;
+            }
+        }
+        used |= 1u << bj;
+        p |= (uint64_t)bj << (4 * i);
+    }
+    return p;
+}
+
+// nearest-neighbour map a_i -> argmin_j |b_j - a_i|; conflict bit when not a bijection
+template <int DEG>
+__device__ __forceinline__ uint64_t nn_map(const cd (&a)[DEG], const cd (&b)[DEG]) {
+    unsigned hit = 0;
+    bool conflict = false;
+    uint64_t p = 0;
+#pragma unroll
+    for (int i = 0; i < DEG; ++i) {
+        double best = norm2(b[0] - a[i]);
+        int bj = 0;
+#pragma unroll
+        for (int j = 1; j < DEG; ++j) {
+            const double d = norm2(b[j] - a[i]);
+            if (d < best) {
+                best = d;
+                bj = j;
+            }
+        }
+        conflict = conflict || ((hit >> bj) & 1u);
+        hit |= 1u << bj;
+        p |= (uint64_t)bj << (4 * i);
+    }
+    return conflict ? (p | kConflictBit) : p;
+}
+
+// ---- point solve -----------------------------------------------------------
+template <int NL, int MODE>
+__device__ __forceinline__ bool ext_solve_point(cd w, const LensFrame<NL>& lf,
+                                                cd (&z)[ext_degree<NL>()], uint32_t& flags,
+                                                int itmax) {
+    constexpr int DEG = ext_degree<NL>();
+    bool ok = true;
+    flags = 0;
+    if constexpr (NL == 1) {
+        const double sq = sqrt(1.0 + 4.0 * rcp(norm2(w)));
+        z[0] = (0.5 * (1.0 + sq)) * w;
+        z[1] = (0.5 * (1.0 - sq)) * w;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            flags |= 1u << j;
+            // sign of det J; the reference's 1 - 1/|z|^2 has the same sign as 1 - 1/|z|^4
+            if (norm2(z[j]) < 1.0) flags |= 1u << (16 + j);
+        }
+    } else {
+        ok = solve_lens<NL, MODE>(w, lf, z, itmax);
+#pragma unroll
+        for (int j = 0; j < DEG; ++j) {
+            ImageTerms<NL> it = image_terms<NL>(z[j], lf);
+            if (norm2(z[j] - it.G - w) < 1e-12) flags |= 1u << j;
+            if (1.0 - norm2(it.K) < 0.0) flags |= 1u << (16 + j);
+        }
+    }
+    return ok;
+}
+
+template <int DEG>
+__device__ __forceinline__ void load_pt(const cd* zb, int k, cd (&z)[DEG]) {
+#pragma unroll
+    for (int r = 0; r < DEG; ++r) z[r] = zb[(size_t)k * DEG + r];
+}
+template <int DEG>
+__device__ __forceinline__ void store_pt(cd* zb, int k, const cd (&z)[DEG]) {
+#pragma unroll
+    for (int r = 0; r < DEG; ++r) zb[(size_t)k * DEG + r] = z[r];
+}
+template <int DEG>
+__device__ __forceinline__ uint32_t permute_flags(uint32_t f, uint64_t p) {
+    uint32_t o = 0;
+#pragma unroll
+    for (int i = 0; i < DEG; ++i) {
+        const int s = pget(p, i);
+        o |= ((f >> s) & 1u) << i;
+        o |= ((f >> (16 + s)) & 1u) << (16 + i);
+    }
+    return o;
+}
+template <int DEG>
+__device__ __forceinline__ void permute_pt(cd (&z)[DEG], uint64_t p) {
+    cd t[DEG];
+#pragma unroll
+    for (int i = 0; i < DEG; ++i) {
+        cd v = z[0];
+#pragma unroll
+        for (int j = 1; j < DEG; ++j) v = (pget(p, i) == j) ? z[j] : v;
+        t[i] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < DEG; ++i) z[i] = t[i];
+}
+
+// _permute_images over points 0..M-1 of buffer b (extended_source.py:34-53)
+template <int DEG>
+__device__ void chain_images(cd* zb, uint32_t* fb, uint64_t* perm, int M) {
+    const int tid = threadIdx.x;
+    for (int k = tid; k < M - 1; k += kExtBlock) {
+        cd a[DEG], b[DEG];
+        load_pt<DEG>(zb, k, a);
+        load_pt<DEG>(zb, k + 1, b);
+        perm[k] = nn_map<DEG>(a, b);
+    }
+    __syncthreads();
+    if (tid < 64) {
+        // wave 0: sequential composition, 64 steps per batch, uniform control flow
+        uint64_t pi = pident<DEG>();
+        for (int base = 0; base < M; base += 64) {
+            const int k_l = base + tid;
+            const uint64_t v = (k_l < M - 1) ? perm[k_l] : 0ull;
+            uint64_t mine = 0;
+            const int nb = min(64, M - base);
+            for (int j = 0; j < nb; ++j) {
+                const int k = base + j;
+                const uint64_t nn = __shfl(v, j, 64);
+                if (tid == j) mine = pi;
+                if (k < M - 1) {
+                    if (!(nn & kConflictBit)) {
+                        pi = pcompose<DEG>(nn, pi);
+                    } else {
+                        cd a[DEG], raw[DEG], b[DEG];
+                        load_pt<DEG>(zb, k, raw);
+#pragma unroll
+                        for (int i = 0; i < DEG; ++i) {
+                            cd t = raw[0];
+#pragma unroll
+                            for (int q = 1; q < DEG; ++q) t = (pget(pi, i) == q) ? raw[q] : t;
+                            a[i] = t;
+                        }
+                        load_pt<DEG>(zb, k + 1, b);
+                        pi = greedy_match<DEG>(a, b);
+                    }
+                }
+            }
+            if (k_l < M) perm[k_l] = mine;
+        }
+    }
+    __syncthreads();
+    for (int k = tid; k < M; k += kExtBlock) {
+        const uint64_t p = perm[k];
+        cd z[DEG];
+        load_pt<DEG>(zb, k, z);
+        permute_pt<DEG>(z, p);
+        store_pt<DEG>(zb, k, z);
+        fb[k] = permute_flags<DEG>(fb[k], p);
+    }
+    __syncthreads();
+}
+
+// block-wide exclusive prefix sum of per-thread counts (kExtBlock threads)
+__device__ inline int block_exclusive_scan(int v, int* sh, int& total) {
+    const int tid = threadIdx.x;
+    sh[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < kExtBlock; off <<= 1) {
+        const int t = (tid >= off) ? sh[tid - off] : 0;
+        __syncthreads();
+        sh[tid] += t;
+        __syncthreads();
+    }
+    total = sh[kExtBlock - 1];
+    const int r = sh[tid] - v;
+    __syncthreads();
+    return r;
+}
+
+__device__ inline double block_sum(double v, double* sh) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    v = wave_sum(v);
+    if (lane == 0) sh[wv] = v;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < kExtBlock / 64; ++q) s += sh[q];
+    __syncthreads();
+    return s;
+}
+
+// ---- segments and contours ---------------------------------------------------
+struct Seg {
+    int row, start, len;  // points start..start+len-1 of image row `row`
+    int par;              // parity of the segment (+1/-1)
+    int empty;
+    double length;        // sum of |z_{k+1} - z_k|
+};
+struct Piece {
+    int row, start, len, rev;
+};
+struct Contour {
+    Piece pc[kExtMaxPieces];
+    int np;
+    int npts;  // points (without the closing point)
+    int par;   // parity of the first point
+};
+
+template <int DEG>
+__device__ __forceinline__ cd pt(const cd* zb, int k, int row) {
+    return zb[(size_t)k * DEG + row];
+}
+// point k of a piece list
+template <int DEG>
+__device__ inline cd contour_point(const cd* zb, const Piece* pc, int np, int k) {
+    for (int i = 0; i < np; ++i) {
+        if (k < pc[i].len) {
+            const int idx = pc[i].rev ? pc[i].start + pc[i].len - 1 - k : pc[i].start + k;
+            return pt<DEG>(zb, idx, pc[i].row);
+        }
+        k -= pc[i].len;
+    }
+    return cd{0.0, 0.0};
+}
+
+}  // namespace caus
+
+namespace caus {
+
+// Gauss-Legendre nodes/weights on [-1,1] by Newton iteration on P_n (ascending x)
+__device__ inline void gauss_legendre(int n, double* x, double* w) {
+    for (int i = threadIdx.x; i < n; i += kExtBlock) {
+        // root i counted from x = +1 downwards, stored ascending
+        double z = cos(3.14159265358979323846 * (i + 0.75) / (n + 0.5));
+        double pp = 1.0;
+        for (int it = 0; it < 100; ++it) {
+            double p1 = 1.0, p2 = 0.0;
+            for (int j = 1; j <= n; ++j) {
+                const double p3 = p2;
+                p2 = p1;
+                p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+            }
+            pp = n * (z * p1 - p2) / (z * z - 1.0);
+            const double dz = p1 / pp;
+            z -= dz;
+            if (fabs(dz) < 1e-16) break;
+        }
+        double p1 = 1.0, p2 = 0.0;
+        for (int j = 1; j <= n; ++j) {
+            const double p3 = p2;
+            p2 = p1;
+            p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+        }
+        pp = n * (z * p1 - p2) / (z * z - 1.0);
+        x[n - 1 - i] = z;
+        w[n - 1 - i] = 2.0 / ((1.0 - z * z) * pp * pp);
+    }
+}
+
+template <int NL>
+__device__ __forceinline__ double ld_brightness(cd z, const LensFrame<NL>& lf, cd w0, double irho,
+                                                double u1) {
+    cd w = z;
+    const cd zb = conj(z);
+#pragma unroll
+    for (int j = 0; j < NL; ++j) w = w - lf.e[j] * crcp<2>(zb - conj(lf.r[j]));
+    const double r = sqrt(norm2(w - w0)) * irho;
+    double Br;
+    if (r <= 1.0) {
+        Br = 1.0 + sqrt(fmax(1.0 - r * r, 0.0));
+    } else {
+        const double ir = rcp(r);
+        Br = 1.0 - sqrt(fmax(1.0 - ir * ir, 0.0));
+    }
+    return 3.0 / (3.0 - u1) * (u1 * Br + 1.0 - 2.0 * u1);
+}
+
+struct GLNodes {
+    const double *x1, *w1, *x2, *w2;
+    int n1, n2;
+};
+
+// -0.5 * int_{a}^{b} B(x + i y) dy  (P) or 0.5 * int_{a}^{b} B(x + i y) dx (Q), split at
+// the reference's point (integrate.py:50-107)
+template <int NL>
+__device__ __forceinline__ double ld_line(bool isP, double fixed, double a, double b, double rho,
+                                          const LensFrame<NL>& lf, cd w0, double u1,
+                                          const GLNodes& g) {
+    const double irho = rcp(rho);
+    const double d = fabs(b - a);
+    double sp = (b > a) ? b - 2.0 * rho : b + 2.0 * rho;
+    if (0.5 * d <= 2.0 * rho) sp = a + 0.5 * d;
+    double I = 0.0;
+    for (int part = 0; part < 2; ++part) {
+        const double lo = part == 0 ? a : sp, hi = part == 0 ? sp : b;
+        const double* x = part == 0 ? g.x1 : g.x2;
+        const double* wt = part == 0 ? g.w1 : g.w2;
+        const int n = part == 0 ? g.n1 : g.n2;
+        const double h = 0.5 * (hi - lo), c = 0.5 * (hi + lo);
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) {
+            const double v = fma(h, x[i], c);
+            const cd zz = isP ? cd{fixed, v} : cd{v, fixed};
+            s = fma(h * ld_brightness<NL>(zz, lf, w0, irho, u1), wt[i], s);
+        }
+        I += s;
+    }
+    return isP ? -0.5 * I : 0.5 * I;
+}
+
+// Contour integral over points c_0..c_{L-1} (pieces), closed back to c_0.
+// nterms trapezoid terms (L, or L-1 for the single-lens quirk); `cdiv` divides the
+// centroid sum (which includes the closing point) for limb darkening.
+template <int NL, int DEG, bool LD>
+__device__ double integrate_contour(const cd* zb, const Piece* pc, int np, int L, int nterms,
+                                    double cdiv, double* pq, const LensFrame<NL>& lf, cd w0,
+                                    double rho, double u1, const GLNodes& g, double* shd) {
+    const int tid = threadIdx.x;
+    double acc = 0.0;
+    if (L <= 0) {
+        return block_sum(0.0, shd);
+    }
+    if constexpr (!LD) {
+        for (int k = tid; k < nterms; k += kExtBlock) {
+            const cd a = contour_point<DEG>(zb, pc, np, k);
+            const cd b = contour_point<DEG>(zb, pc, np, (k + 1 == L) ? 0 : k + 1);
+            acc += 0.5 * (0.5 * a.re + 0.5 * b.re) * (b.im - a.im) +
+                   0.5 * (-0.5 * a.im - 0.5 * b.im) * (b.re - a.re);
+        }
+        return block_sum(acc, shd);
+    } else {
+        double sx = 0.0, sy = 0.0;
+        for (int k = tid; k < L; k += kExtBlock) {
+            const cd a = contour_point<DEG>(zb, pc, np, k);
+            sx += a.re;
+            sy += a.im;
+        }
+        const cd c0 = contour_point<DEG>(zb, pc, np, 0);
+        const double zx = (block_sum(sx, shd) + c0.re) / cdiv;
+        const double zy = (block_sum(sy, shd) + c0.im) / cdiv;
+        for (int k = tid; k < L; k += kExtBlock) {
+            const cd a = contour_point<DEG>(zb, pc, np, k);
+            pq[2 * k] = ld_line<NL>(true, a.re, zy, a.im, rho, lf, w0, u1, g);
+            pq[2 * k + 1] = ld_line<NL>(false, a.im, zx, a.re, rho, lf, w0, u1, g);
+        }
+        __syncthreads();
+        for (int k = tid; k < nterms; k += kExtBlock) {
+            const int k1 = (k + 1 == L) ? 0 : k + 1;
+            const cd a = contour_point<DEG>(zb, pc, np, k);
+            const cd b = contour_point<DEG>(zb, pc, np, k1);
+            acc += 0.5 * (pq[2 * k] + pq[2 * k1]) * (b.re - a.re) +
+                   0.5 * (pq[2 * k + 1] + pq[2 * k1 + 1]) * (b.im - a.im);
+        }
+        const double r = block_sum(acc, shd);
+        __syncthreads();
+        return r;
+    }
+}
+
+// ---- stitching (extended_source.py:312-667), wave-uniform code on wave 0 ---------
+struct ActSeg {
+    Piece pc[kExtMaxPieces];
+    int np, t, par, empty;
+};
+
+template <int DEG>
+__device__ inline cd seg_point(const cd* zb, const Seg& s, int k) {
+    if (s.empty || k < 0 || k >= s.len) return cd{0.0, 0.0};
+    return pt<DEG>(zb, s.start + k, s.row);
+}
+template <int DEG>
+__device__ inline cd act_point(const cd* zb, const ActSeg& a, int k) {
+    if (a.empty || k < 0 || k > a.t) return cd{0.0, 0.0};
+    return contour_point<DEG>(zb, a.pc, a.np, k);
+}
+
+// line ending at the head / tail (second element is the connection point).
+// Dense-array semantics: points beyond the segment are zero, dynamic_slice clamps.
+template <class F>
+__device__ inline void head_line(F get, int t, cd& l0, cd& l1) {
+    const cd x0 = get(0), x1 = get(1), x2 = get(2);
+    const bool c = (norm2(x1 - x0) > 1e-10) || (t <= 1);
+    if (c) {
+        l0 = x1;
+        l1 = x0;
+    } else {
+        l0 = x2;
+        l1 = x1;
+    }
+}
+template <class F>
+__device__ inline void tail_line(F get, int t, int dense_len, cd& l0, cd& l1) {
+    const bool c = (t <= 1) || (norm2(get(t) - get(t - 1)) > 1e-10);
+    int s = c ? t - 1 : t - 2;
+    s = s < 0 ? 0 : (s > dense_len - 2 ? dense_len - 2 : s);
+    l0 = get(s);
+    l1 = get(s + 1);
+}
+
+__device__ inline bool geom_condition(cd a0, cd a1, cd b0, cd b1) {
+    const double dist = sqrt(norm2(a1 - b1));
+    const bool c1 = dist < 1e-1;
+    const cd v1 = a1 - a0, v2 = b1 - b0;
+    const double n1 = sqrt(norm2(v1)), n2 = sqrt(norm2(v2));
+    const double cosang = (v1.re / n1) * (v2.re / n2) + (v1.im / n1) * (v2.im / n2);
+    const double alpha = acos(cosang);  // NaN outside [-1,1] or for a zero vector
+    const bool c2 = (180.0 - alpha * (180.0 / 3.14159265358979323846)) < 60.0;
+    const bool c3 = sqrt(norm2(a1 - b1)) < sqrt(norm2(a0 - b0));
+    return (c1 && c2 && c3) || dist < 1e-5;
+}
+
+}  // namespace caus
+
+// ============================================================================
+// Reverse mode of the contour pipeline.  Directions (user order):
+//   0 Re w0, 1 Im w0, 2 rho, 3 u1, 4.. lens parameters (s, q[, q3, r3, psi]).
+// Discrete choices (limb angles, rows, segment structure) are piecewise constant;
+// every contour point is the image z of a fixed limb angle, differentiated
+// implicitly through the lens equation, and the Gauss-Legendre line integrals are
+// differentiated exactly as discretised (moving end points and split point included).
+// ============================================================================
+namespace caus {
+
+template <int NL>
+constexpr int ext_nd() {
+    return 4 + (NL == 1 ? 0 : n_user_params<NL>());
+}
+
+template <int NL>
+struct ExtTangents {
+    static constexpr int ND = ext_nd<NL>();
+    cd dw0[ND];              // tangent of the lens-frame source centre
+    LensTangent<NL> lt[ND];  // tangent of the lens frame
+    double drho[ND], du1[ND];
+};
+
+template <int NL>
+__device__ inline void ext_tangents(const double* prm, ExtTangents<NL>& T) {
+    constexpr int ND = ext_nd<NL>();
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+#pragma unroll
+        for (int j = 0; j < NL; ++j) {
+            T.lt[d].de[j] = 0.0;
+            T.lt[d].dr[j] = cd{0.0, 0.0};
+        }
+        T.lt[d].dxcm = 0.0;
+        T.dw0[d] = cd{0.0, 0.0};
+        T.drho[d] = 0.0;
+        T.du1[d] = 0.0;
+    }
+    T.dw0[0] = cd{1.0, 0.0};
+    T.dw0[1] = cd{0.0, 1.0};
+    T.drho[2] = 1.0;
+    T.du1[3] = 1.0;
+    if constexpr (NL > 1) {
+        LensTangent<NL> up[n_user_params<NL>()];
+        lens_frame_tangents<NL>(prm, up);
+#pragma unroll
+        for (int k = 0; k < n_user_params<NL>(); ++k) {
+            T.lt[4 + k] = up[k];
+            T.dw0[4 + k] = cd{up[k].dxcm, 0.0};
+        }
+    }
+}
+
+// point k of a piece list: value, limb index and row
+template <int DEG>
+__device__ inline cd contour_locate(const cd* zb, const Piece* pc, int np, int k, int& idx,
+                                    int& row) {
+    for (int i = 0; i < np; ++i) {
+        if (k < pc[i].len) {
+            idx = pc[i].rev ? pc[i].start + pc[i].len - 1 - k : pc[i].start + k;
+            row = pc[i].row;
+            return pt<DEG>(zb, idx, row);
+        }
+        k -= pc[i].len;
+    }
+    idx = 0;
+    row = 0;
+    return cd{0.0, 0.0};
+}
+
+// Tangents of the image z (limb angle th) along every direction.
+template <int NL, int ND>
+__device__ inline void image_tangents(cd z, double th, const LensFrame<NL>& lf,
+                                      const ExtTangents<NL>& T, cd (&dz)[ND]) {
+    const ImageTerms<NL> it = image_terms<NL>(z, lf);
+    const double det = 1.0 - norm2(it.K);
+    const double idet = rcp(det);
+    double sn, cs;
+    sincos(th, &sn, &cs);
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        cd u = T.dw0[d] + T.drho[d] * cd{cs, sn};
+#pragma unroll
+        for (int j = 0; j < NL; ++j) {
+            const cd v = it.inv[j];
+            u = u + T.lt[d].de[j] * v;
+            u = cfma(lf.e[j] * (v * v), conj(T.lt[d].dr[j]), u);
+        }
+        dz[d] = idet * (u - it.K * conj(u));
+    }
+}
+
+// brightness and its derivatives at z: position (Bx, By) and direct directions
+template <int NL, int ND>
+__device__ inline double ld_brightness_grad(cd z, const LensFrame<NL>& lf, cd w0, double rho,
+                                            double u1, const ExtTangents<NL>& T, double& Bx,
+                                            double& By, double (&Bd)[ND]) {
+    const cd zb = conj(z);
+    cd G = cd{0.0, 0.0}, K = cd{0.0, 0.0};
+    cd v[NL];
+#pragma unroll
+    for (int j = 0; j < NL; ++j) {
+        v[j] = crcp<2>(zb - conj(lf.r[j]));
+        G = G + lf.e[j] * v[j];
+        K = K + lf.e[j] * (v[j] * v[j]);
+    }
+    const cd D = z - G - w0;
+    const double aD = sqrt(norm2(D));
+    const double irho = rcp(rho);
+    const double r = aD * irho;
+    double Br, dBr;
+    if (r <= 1.0) {
+        const double x = 1.0 - r * r;
+        Br = 1.0 + sqrt(fmax(x, 0.0));
+        dBr = x > 0.0 ? -r / sqrt(x) : 0.0;
+    } else {
+        const double ir = rcp(r);
+        const double x = 1.0 - ir * ir;
+        Br = 1.0 - sqrt(fmax(x, 0.0));
+        dBr = x > 0.0 ? -(ir * ir * ir) / sqrt(x) : 0.0;
+    }
+    const double c3 = 3.0 / (3.0 - u1);
+    const double B = c3 * (u1 * Br + 1.0 - 2.0 * u1);
+    const double dBdr = c3 * u1 * dBr;
+    const double iDr = (aD > 0.0) ? rcp(aD * rho) : 0.0;
+    // dr = Re(conj(D) dD)/(|D| rho) - r drho/rho
+    const cd dDx = cd{1.0, 0.0} + K;
+    const cd dDy = cd{0.0, 1.0} * (cd{1.0, 0.0} - K);
+    Bx = dBdr * (D.re * dDx.re + D.im * dDx.im) * iDr;
+    By = dBdr * (D.re * dDy.re + D.im * dDy.im) * iDr;
+    const double dBdu1 = c3 * c3 / 3.0 * (u1 * Br + 1.0 - 2.0 * u1) + c3 * (Br - 2.0);
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        cd dG = cd{0.0, 0.0};
+#pragma unroll
+        for (int j = 0; j < NL; ++j) {
+            dG = dG + T.lt[d].de[j] * v[j];
+            dG = cfma(lf.e[j] * (v[j] * v[j]), conj(T.lt[d].dr[j]), dG);
+        }
+        const cd dD = -1.0 * (dG + T.dw0[d]);
+        const double dr = (D.re * dD.re + D.im * dD.im) * iDr - r * T.drho[d] * irho;
+        Bd[d] = dBdr * dr + dBdu1 * T.du1[d];
+    }
+    return B;
+}
+
+// P (isP) or Q line integral with its partial derivatives: d/d(fixed coordinate),
+// d/d(lower limit a), d/d(upper limit b), and direct directions.
+template <int NL, int ND>
+__device__ inline double ld_line_grad(bool isP, double fixed, double a, double b, double rho,
+                                      const LensFrame<NL>& lf, cd w0, double u1,
+                                      const GLNodes& g, const ExtTangents<NL>& T, double& dfix,
+                                      double& da, double& db, double (&dth)[ND]) {
+    const double d = fabs(b - a);
+    double sp, dsp_da, dsp_db, dsp_drho;
+    if (0.5 * d <= 2.0 * rho) {
+        const double sg = (b - a) > 0 ? 1.0 : ((b - a) < 0 ? -1.0 : 0.0);
+        sp = a + 0.5 * d;
+        dsp_da = 1.0 - 0.5 * sg;
+        dsp_db = 0.5 * sg;
+        dsp_drho = 0.0;
+    } else if (b > a) {
+        sp = b - 2.0 * rho;
+        dsp_da = 0.0;
+        dsp_db = 1.0;
+        dsp_drho = -2.0;
+    } else {
+        sp = b + 2.0 * rho;
+        dsp_da = 0.0;
+        dsp_db = 1.0;
+        dsp_drho = 2.0;
+    }
+    double S = 0.0, dS_lo[2], dS_hi[2];
+    dfix = 0.0;
+#pragma unroll
+    for (int q = 0; q < ND; ++q) dth[q] = 0.0;
+    for (int part = 0; part < 2; ++part) {
+        const double lo = part == 0 ? a : sp, hi = part == 0 ? sp : b;
+        const double* x = part == 0 ? g.x1 : g.x2;
+        const double* wt = part == 0 ? g.w1 : g.w2;
+        const int n = part == 0 ? g.n1 : g.n2;
+        const double h = 0.5 * (hi - lo), c = 0.5 * (hi + lo);
+        double s = 0.0, slo = 0.0, shi = 0.0;
+        for (int i = 0; i < n; ++i) {
+            const double t = fma(h, x[i], c);
+            const cd zz = isP ? cd{fixed, t} : cd{t, fixed};
+            double Bx, By, Bd[ND];
+            const double B = ld_brightness_grad<NL, ND>(zz, lf, w0, rho, u1, T, Bx, By, Bd);
+            const double fl = isP ? By : Bx;   // along the line
+            const double ff = isP ? Bx : By;   // across (fixed coordinate)
+            s = fma(h * B, wt[i], s);
+            slo += wt[i] * (-0.5 * B + 0.5 * h * fl * (1.0 - x[i]));
+            shi += wt[i] * (0.5 * B + 0.5 * h * fl * (1.0 + x[i]));
+            dfix += h * wt[i] * ff;
+#pragma unroll
+            for (int q = 0; q < ND; ++q) dth[q] += h * wt[i] * Bd[q];
+        }
+        S += s;
+        dS_lo[part] = slo;
+        dS_hi[part] = shi;
+    }
+    const double dsp = dS_hi[0] + dS_lo[1];
+    da = dS_lo[0] + dsp * dsp_da;
+    db = dS_hi[1] + dsp * dsp_db;
+#pragma unroll
+    for (int q = 0; q < ND; ++q) dth[q] += dsp * dsp_drho * T.drho[q];
+    const double sc = isP ? -0.5 : 0.5;
+    dfix *= sc;
+    da *= sc;
+    db *= sc;
+#pragma unroll
+    for (int q = 0; q < ND; ++q) dth[q] *= sc;
+    return sc * S;
+}
+
+// Adjoint of one contour integral: accumulates scale * dI/dtheta into g[ND] (per thread).
+// pq holds P, Q of the contour points (limb darkening; written by the forward pass).
+template <int NL, int DEG, bool LD, int ND>
+__device__ void contour_adjoint(const cd* zb, const double* theta, const Piece* pc, int np, int L,
+                                int nterms, double cdiv, const double* pq,
+                                const LensFrame<NL>& lf, cd w0, double rho, double u1,
+                                const GLNodes& gl, const ExtTangents<NL>& T, double scale,
+                                double (&g)[ND], double* shd) {
+    const int tid = threadIdx.x;
+    if (L <= 0) return;
+    double zx = 0.0, zy = 0.0;
+    if constexpr (LD) {
+        double sx = 0.0, sy = 0.0;
+        for (int k = tid; k < L; k += kExtBlock) {
+            const cd a = contour_point<DEG>(zb, pc, np, k);
+            sx += a.re;
+            sy += a.im;
+        }
+        const cd c0 = contour_point<DEG>(zb, pc, np, 0);
+        zx = (block_sum(sx, shd) + c0.re) / cdiv;
+        zy = (block_sum(sy, shd) + c0.im) / cdiv;
+    }
+    double x0bar = 0.0, y0bar = 0.0;   // adjoints of the centroid (per thread)
+    cd sdz[ND];                        // sum of dz over points, weighted for the centroid
+#pragma unroll
+    for (int d = 0; d < ND; ++d) sdz[d] = cd{0.0, 0.0};
+    for (int k = tid; k < L; k += kExtBlock) {
+        int idx, row;
+        const cd zk = contour_locate<DEG>(zb, pc, np, k, idx, row);
+        const int kn = (k + 1 == L) ? 0 : k + 1;
+        const int kp = (k == 0) ? L - 1 : k - 1;
+        const bool t_first = k < nterms;    // term k: (k, kn)
+        const bool t_second = kp < nterms;  // term kp: (kp, k)
+        const cd zn = contour_point<DEG>(zb, pc, np, kn);
+        const cd zp = contour_point<DEG>(zb, pc, np, kp);
+        double xbar = 0.0, ybar = 0.0;
+        double thd[ND];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) thd[d] = 0.0;
+        if constexpr (!LD) {
+            if (t_first) {
+                xbar += 0.5 * zn.im;
+                ybar -= 0.5 * zn.re;
+            }
+            if (t_second) {
+                xbar -= 0.5 * zp.im;
+                ybar += 0.5 * zp.re;
+            }
+        } else {
+            const double Pk = pq[2 * k], Qk = pq[2 * k + 1];
+            const double Pn = pq[2 * kn], Qn = pq[2 * kn + 1];
+            const double Pp = pq[2 * kp], Qp = pq[2 * kp + 1];
+            double Pbar = 0.0, Qbar = 0.0;
+            if (t_first) {
+                Pbar += 0.5 * (zn.re - zk.re);
+                Qbar += 0.5 * (zn.im - zk.im);
+                xbar -= 0.5 * (Pk + Pn);
+                ybar -= 0.5 * (Qk + Qn);
+            }
+            if (t_second) {
+                Pbar += 0.5 * (zk.re - zp.re);
+                Qbar += 0.5 * (zk.im - zp.im);
+                xbar += 0.5 * (Pp + Pk);
+                ybar += 0.5 * (Qp + Qk);
+            }
+            double dfix, da, db, dth[ND];
+            // P_k = P(x_k; y0 .. y_k)
+            ld_line_grad<NL, ND>(true, zk.re, zy, zk.im, rho, lf, w0, u1, gl, T, dfix, da, db, dth);
+            xbar += Pbar * dfix;
+            ybar += Pbar * db;
+            y0bar += Pbar * da;
+#pragma unroll
+            for (int d = 0; d < ND; ++d) thd[d] += Pbar * dth[d];
+            // Q_k = Q(y_k; x0 .. x_k)
+            ld_line_grad<NL, ND>(false, zk.im, zx, zk.re, rho, lf, w0, u1, gl, T, dfix, da, db, dth);
+            ybar += Qbar * dfix;
+            xbar += Qbar * db;
+            x0bar += Qbar * da;
+#pragma unroll
+            for (int d = 0; d < ND; ++d) thd[d] += Qbar * dth[d];
+        }
+        cd dz[ND];
+        image_tangents<NL, ND>(zk, theta[idx], lf, T, dz);
+        const double wgt = (k == 0) ? 2.0 : 1.0;  // the closing point repeats c_0 in the centroid
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            g[d] += scale * (xbar * dz[d].re + ybar * dz[d].im + thd[d]);
+            sdz[d] = sdz[d] + wgt * dz[d];
+        }
+    }
+    if constexpr (LD) {
+        const double X0 = block_sum(x0bar, shd) / cdiv;
+        const double Y0 = block_sum(y0bar, shd) / cdiv;
+#pragma unroll
+        for (int d = 0; d < ND; ++d) g[d] += scale * (X0 * sdz[d].re + Y0 * sdz[d].im);
+    }
+}
+
+}  // namespace caus
+
+namespace caus {
+
+struct ExtArgs {
+    const cd* w0;
+    int64_t S;
+    const double* rho;
+    const double* params;
+    int64_t pstride;
+    const int32_t* npts;
+    int npts_def;
+    const double* u1;
+    int64_t u1stride;
+    int npts_ld;
+    double* mag;
+    uint8_t* status;
+    const double* g_mag;   // VJP only
+    double* g_out;         // VJP only: (S, 4 + P)
+    char* ws;
+    size_t ws_per_source;
+    int mmax;
+    int itmax;
+};
+
+template <int NL, int MODE, bool LD, bool VJP>
+__global__ __launch_bounds__(kExtBlock) void k_mag_extended(ExtArgs A) {
+    constexpr int DEG = ext_degree<NL>();
+    constexpr int ND = ext_nd<NL>();
+    constexpr int NIMG = NL * NL + 1;
+    constexpr int NSEG = 3 * NIMG;   // nr_of_segments
+    __shared__ uint64_t s_key[kExtSortMax];
+    __shared__ int s_idx[kExtSortMax];
+    __shared__ int s_int[kExtBlock];
+    __shared__ double s_d[8];
+    __shared__ double s_gl[4 * 128];
+    __shared__ int s_fail;
+    __shared__ Seg s_seg[kExtMaxSeg];
+    __shared__ int s_nseg;
+    __shared__ ActSeg s_con[3];
+    __shared__ unsigned s_allmask;
+
+    const int tid = threadIdx.x;
+    const int64_t src = blockIdx.x;
+    if (src >= A.S) return;
+    const double* prm = A.params + src * A.pstride;
+    const double rho = A.rho[src];
+    const double u1 = LD ? A.u1[src * A.u1stride] : 0.0;
+    LensFrame<NL> lf;
+    double x_cm = 0.0;
+    if constexpr (NL == 1) {
+        lf.e[0] = 1.0;
+        lf.r[0] = cd{0.0, 0.0};
+    } else {
+        lens_frame<NL>(prm, lf, x_cm);
+    }
+    const cd w0 = A.w0[src] + cd{x_cm, 0.0};
+    int npts = A.npts ? A.npts[src] : A.npts_def;
+    bool bad_npts = false;
+    if (npts < 4 || npts > A.npts_def) {  // the workspace is sized for npts_def
+        bad_npts = true;
+        npts = npts < 4 ? 4 : A.npts_def;
+    }
+    const int M0 = ext_npts_init(npts), nadd = ext_n_add(npts);
+    ExtWS W = ext_ws(A.ws + (size_t)src * A.ws_per_source, DEG, A.mmax);
+    if (tid == 0) s_fail = bad_npts ? 1 : 0;
+    GLNodes g{};
+    if constexpr (LD) {
+        g.n1 = A.npts_ld / 2;
+        g.n2 = A.npts_ld - g.n1;
+        g.x1 = s_gl;
+        g.w1 = s_gl + 128;
+        g.x2 = s_gl + 256;
+        g.w2 = s_gl + 384;
+        gauss_legendre(g.n1, s_gl, s_gl + 128);
+        gauss_legendre(g.n2, s_gl + 256, s_gl + 384);
+    }
+    __syncthreads();
+
+    // ---- initial limb points (extended_source.py:100-106)
+    const double PI = 3.14159265358979323846;
+    for (int k = tid; k < M0; k += kExtBlock) {
+        const double th = (k < M0 - 1) ? -PI + k * (2.0 * PI / (M0 - 1)) : PI - 1e-8;
+        double sn, cs;
+        sincos(th, &sn, &cs);
+        cd z[DEG];
+        uint32_t f;
+        if (!ext_solve_point<NL, MODE>(w0 + rho * cd{cs, sn}, lf, z, f, A.itmax)) s_fail = 1;
+        W.theta[0][k] = th;
+        store_pt<DEG>(W.z[0], k, z);
+        W.flg[0][k] = f;
+    }
+    __syncthreads();
+    chain_images<DEG>(W.z[0], W.flg[0], W.perm, M0);
+
+    // ---- refinement rounds (extended_source.py:108-135)
+    int cur = 0, M = M0;
+    for (int round = 0; round < 10; ++round) {
+        const cd* zb = W.z[cur];
+        const uint32_t* fb = W.flg[cur];
+        int P2 = 1;
+        while (P2 < M - 1) P2 <<= 1;
+        for (int k = tid; k < P2; k += kExtBlock) {
+            uint64_t key = 0;
+            int idx = -1;
+            if (k < M - 1) {
+                cd a[DEG], b[DEG];
+                load_pt<DEG>(zb, k, a);
+                load_pt<DEG>(zb, k + 1, b);
+                const uint32_t m = (fb[k] | fb[k + 1]) & 0xFFFFu;
+                double dmax = 0.0;
+#pragma unroll
+                for (int r = 0; r < DEG; ++r)
+                    if ((m >> r) & 1u) dmax = fmax(dmax, sqrt(norm2(b[r] - a[r])));
+                key = (uint64_t)__double_as_longlong(dmax);
+                idx = k;
+            }
+            s_key[k] = key;
+            s_idx[k] = idx;
+        }
+        __syncthreads();
+        // bitonic sort, descending by (key, idx)
+        for (int size = 2; size <= P2; size <<= 1) {
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                for (int i = tid; i < P2; i += kExtBlock) {
+                    const int j = i ^ stride;
+                    if (j > i) {
+                        const bool desc = (i & size) == 0;
+                        const uint64_t ki = s_key[i], kj = s_key[j];
+                        const int ii = s_idx[i], ij = s_idx[j];
+                        const bool i_less = (ki < kj) || (ki == kj && ii < ij);
+                        if (desc == i_less) {
+                            s_key[i] = kj;
+                            s_key[j] = ki;
+                            s_idx[i] = ij;
+                            s_idx[j] = ii;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        // mark the selected intervals; exclusive prefix count -> shifts (perm[] as ints)
+        int* shift = reinterpret_cast<int*>(W.perm);
+        const int per = (M + kExtBlock - 1) / kExtBlock;
+        const int k0 = tid * per, k1 = min(M, k0 + per);
+        for (int k = k0; k < k1; ++k) shift[k] = 0;
+        __syncthreads();
+        for (int c = tid; c < nadd; c += kExtBlock) shift[s_idx[c]] = 1;
+        __syncthreads();
+        int cnt = 0;
+        for (int k = k0; k < k1; ++k) cnt += shift[k];
+        int total;
+        int run = block_exclusive_scan(cnt, s_int, total);
+        for (int k = k0; k < k1; ++k) {
+            const int ins = shift[k];
+            shift[k] = run;
+            run += ins;
+        }
+        __syncthreads();
+        const int nxt = cur ^ 1;
+        // move old points
+        for (int k = tid; k < M; k += kExtBlock) {
+            const int d = k + shift[k];
+            cd z[DEG];
+            load_pt<DEG>(zb, k, z);
+            store_pt<DEG>(W.z[nxt], d, z);
+            W.flg[nxt][d] = fb[k];
+            W.theta[nxt][d] = W.theta[cur][k];
+        }
+        // new points, matched to their left neighbour
+        for (int c = tid; c < nadd; c += kExtBlock) {
+            const int k = s_idx[c];
+            const double th = 0.5 * (W.theta[cur][k] + W.theta[cur][k + 1]);
+            double sn, cs;
+            sincos(th, &sn, &cs);
+            cd z[DEG], left[DEG];
+            uint32_t f;
+            if (!ext_solve_point<NL, MODE>(w0 + rho * cd{cs, sn}, lf, z, f, A.itmax)) s_fail = 1;
+            load_pt<DEG>(zb, k, left);
+            const uint64_t p = greedy_match<DEG>(left, z);
+            permute_pt<DEG>(z, p);
+            f = permute_flags<DEG>(f, p);
+            const int d = k + shift[k] + 1;
+            store_pt<DEG>(W.z[nxt], d, z);
+            W.flg[nxt][d] = f;
+            W.theta[nxt][d] = th;
+        }
+        __syncthreads();
+        cur = nxt;
+        M += nadd;
+    }
+    chain_images<DEG>(W.z[cur], W.flg[cur], W.perm, M);
+
+    const cd* zb = W.z[cur];
+    const uint32_t* fb = W.flg[cur];
+    const double* thb = W.theta[cur];
+    double total = 0.0;
+    double gacc[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) gacc[d] = 0.0;
+    ExtTangents<NL> T;
+    if constexpr (VJP) ext_tangents<NL>(prm, T);
+    auto do_contour = [&](const Piece* pcs, int npc, int L, int nterms, double cdiv, int par) {
+        const double I = integrate_contour<NL, DEG, LD>(zb, pcs, npc, L, nterms, cdiv, W.pq, lf,
+                                                        w0, rho, u1, g, s_d);
+        total += par * I;
+        if constexpr (VJP)
+            contour_adjoint<NL, DEG, LD, ND>(zb, thb, pcs, npc, L, nterms, cdiv, W.pq, lf, w0, rho,
+                                             u1, g, T, (double)par, gacc, s_d);
+    };
+    if constexpr (NL == 1) {
+        // both images, tail index N-1 (extended_source.py:860-866)
+        for (int r = 0; r < 2; ++r) {
+            Piece pc{r, 0, M, 0};
+            const int par = ((fb[0] >> (16 + r)) & 1u) ? -1 : 1;
+            do_contour(&pc, 1, M, M - 1, (double)M, par);
+        }
+    } else {
+        // closure test (extended_source.py:289-291)
+        if (tid == 0) s_allmask = 0xFFFFu;
+        __syncthreads();
+        unsigned am = 0xFFFFu;
+        for (int k = tid; k < M; k += kExtBlock) am &= fb[k];
+        atomicAnd(&s_allmask, am);
+        __syncthreads();
+        unsigned closed = 0;
+#pragma unroll
+        for (int r = 0; r < DEG; ++r) {
+            if ((s_allmask >> r) & 1u) {
+                if (norm2(pt<DEG>(zb, 0, r) - pt<DEG>(zb, M - 1, r)) < 1e-10) closed |= 1u << r;
+            }
+        }
+        for (int r = 0; r < DEG; ++r) {
+            if ((closed >> r) & 1u) {
+                Piece pc{r, 0, M, 0};
+                const int par = ((fb[0] >> (16 + r)) & 1u) ? -1 : 1;
+                do_contour(&pc, 1, M, M, (double)(M + 1), par);
+            }
+        }
+        if (closed != (1u << DEG) - 1u) {
+            // changepoints per boundary c in [0, M] (extended_source.py:163-188); rows
+            // that are closed contribute nothing (their segments were zeroed)
+            const unsigned open = ((1u << DEG) - 1u) & ~closed;
+            uint64_t* cpw = W.perm;
+            for (int c = tid; c <= M; c += kExtBlock) {
+                uint64_t word = 0;
+#pragma unroll
+                for (int r = 0; r < DEG; ++r) {
+                    if (!((open >> r) & 1u)) continue;
+                    const int mp = (c > 0) ? (int)((fb[c - 1] >> r) & 1u) : 0;
+                    const int mc = (c < M) ? (int)((fb[c] >> r) & 1u) : 0;
+                    bool cp;
+                    int dm;
+                    if (c == 0) {
+                        cp = mc != 0;
+                        dm = mc ? 1 : 0;
+                    } else if (c == M) {
+                        cp = mp != 0;
+                        dm = mp ? -1 : 0;
+                    } else {
+                        const cd za = mp ? pt<DEG>(zb, c - 1, r) : cd{0.0, 0.0};
+                        const cd zc = mc ? pt<DEG>(zb, c, r) : cd{0.0, 0.0};
+                        const int pa = mp ? ((((fb[c - 1] >> (16 + r)) & 1u) ? -1 : 1)) : 0;
+                        const int pc_ = mc ? ((((fb[c] >> (16 + r)) & 1u) ? -1 : 1)) : 0;
+                        dm = mc - mp;
+                        cp = (norm2(zc - za) > 0.01) || (pc_ != pa) || (dm != 0);
+                    }
+                    if (cp && dm >= 0) word |= 1ull << r;
+                    if (cp && dm <= 0) word |= 1ull << (16 + r);
+                }
+                cpw[c] = word;
+            }
+            __syncthreads();
+            if (tid < 64) {
+                // runs per row (first 10 starts / ends), candidate index row*10+slot;
+                // nonempty (>= 2 points) ones in descending index order (:222-240)
+                Seg cand[kExtMaxSeg];
+                int nc = 0;
+                for (int r = DEG - 1; r >= 0 && nc < NSEG; --r) {
+                    if (!((open >> r) & 1u)) continue;
+                    int st[10], en[10], ns = 0, ne = 0;
+                    for (int c = 0; c <= M; ++c) {
+                        const uint64_t wd = cpw[c];
+                        if (((wd >> r) & 1ull) && ns < 10) st[ns++] = c;
+                        if (((wd >> (16 + r)) & 1ull) && ne < 10) en[ne++] = c;
+                    }
+                    for (int q = 9; q >= 0 && nc < NSEG; --q) {
+                        const int l = q < ns ? st[q] : 0, rr = q < ne ? en[q] : 0;
+                        if (l == 0 && rr == 0) continue;
+                        if (rr - l < 2) continue;
+                        Seg s;
+                        s.row = r;
+                        s.start = l;
+                        s.len = rr - l;
+                        s.par = ((fb[l] >> (16 + r)) & 1u) ? -1 : 1;
+                        s.empty = 0;
+                        double L = 0.0;
+                        for (int k = 0; k + 1 < s.len; ++k)
+                            L += sqrt(norm2(pt<DEG>(zb, l + k + 1, r) - pt<DEG>(zb, l + k, r)));
+                        s.length = L;
+                        cand[nc++] = s;
+                    }
+                }
+                for (int q = nc; q < NSEG; ++q) {
+                    cand[q].empty = 1;
+                    cand[q].len = 0;
+                    cand[q].length = 0.0;
+                    cand[q].par = 0;
+                    cand[q].row = 0;
+                    cand[q].start = 0;
+                }
+                // stitching, 3 contours, merge scans of 20, 18, 16 (:495-667, :690)
+                int R = NSEG;
+                int maxm = 20;
+                for (int ci = 0; ci < 3; ++ci) {
+                    // stable sort by length, empties (length 0) last
+                    for (int a = 1; a < R; ++a) {
+                        Seg key = cand[a];
+                        int b = a - 1;
+                        auto later = [](const Seg& x, const Seg& y) {  // x after y ?
+                            const bool xe = x.empty || x.length == 0.0;
+                            const bool ye = y.empty || y.length == 0.0;
+                            if (xe != ye) return xe;
+                            if (xe) return false;
+                            return x.length > y.length;
+                        };
+                        while (b >= 0 && later(cand[b], key)) {
+                            cand[b + 1] = cand[b];
+                            --b;
+                        }
+                        cand[b + 1] = key;
+                    }
+                    ActSeg act;
+                    act.empty = cand[0].empty;
+                    act.np = 1;
+                    act.pc[0] = Piece{cand[0].row, cand[0].start, cand[0].len, 0};
+                    act.t = cand[0].empty ? 0 : cand[0].len - 1;
+                    act.par = cand[0].par;
+                    for (int q = 1; q < R; ++q) cand[q - 1] = cand[q];
+                    R -= 1;
+                    for (int it = 0; it < maxm; ++it) {
+                        bool any = false;
+                        for (int q = 0; q < R; ++q) any = any || !cand[q].empty;
+                        if (!any) break;
+                        auto ag = [&](int k) { return act_point<DEG>(zb, act, k); };
+                        const cd ah = ag(0), at = ag(act.t);
+                        // 4 nearest (ctype-major flat index, stable)
+                        int bi[4];
+                        double bd[4];
+                        int nb = 0;
+                        for (int ct = 0; ct < 4; ++ct) {
+                            for (int q = 0; q < R; ++q) {
+                                const Seg& s = cand[q];
+                                const cd sh = seg_point<DEG>(zb, s, 0);
+                                const cd stl = seg_point<DEG>(zb, s, s.empty ? 0 : s.len - 1);
+                                const cd d = ct == 0 ? at - sh : ct == 1 ? ah - stl
+                                                               : ct == 2 ? ah - sh : at - stl;
+                                const double dd = sqrt(norm2(d));
+                                const int f = ct * R + q;
+                                int pos = nb;
+                                while (pos > 0 && bd[pos - 1] > dd) --pos;
+                                if (pos < 4) {
+                                    for (int u = min(nb, 3); u > pos; --u) {
+                                        bd[u] = bd[u - 1];
+                                        bi[u] = bi[u - 1];
+                                    }
+                                    bd[pos] = dd;
+                                    bi[pos] = f;
+                                    if (nb < 4) ++nb;
+                                }
+                            }
+                        }
+                        int chosen = -1;
+                        for (int u = 0; u < nb && chosen < 0; ++u) {
+                            const int ct = bi[u] / R, q = bi[u] % R;
+                            const Seg& s = cand[q];
+                            if (s.empty) continue;  // zero segments connect only within 1e-5 of 0
+                            const bool same = act.par * s.par > 0;
+                            const bool okp = (ct < 2) ? same : !same;
+                            if (!okp) continue;
+                            auto sg = [&](int k) { return seg_point<DEG>(zb, s, k); };
+                            const int ts = s.len - 1;
+                            const int dl = 4 * M;
+                            cd a0, a1, b0, b1;
+                            if (ct == 0) {
+                                tail_line(ag, act.t, dl, a0, a1);
+                                head_line(sg, ts, b0, b1);
+                            } else if (ct == 1) {
+                                head_line(ag, act.t, a0, a1);
+                                tail_line(sg, ts, dl, b0, b1);
+                            } else if (ct == 2) {
+                                head_line(ag, act.t, a0, a1);
+                                head_line(sg, ts, b0, b1);
+                            } else {
+                                tail_line(ag, act.t, dl, a0, a1);
+                                tail_line(sg, ts, dl, b0, b1);
+                            }
+                            if (act.empty) {
+                                a0 = a1 = cd{0.0, 0.0};
+                            }
+                            if (geom_condition(a0, a1, b0, b1)) chosen = u;
+                        }
+                        if (chosen < 0) continue;
+                        const int ct = bi[chosen] / R, q = bi[chosen] % R;
+                        const Seg s = cand[q];
+                        const Piece fw{s.row, s.start, s.len, 0}, bw{s.row, s.start, s.len, 1};
+                        if (act.np >= kExtMaxPieces) continue;
+                        if (ct == 0) {          // T-H: act + s
+                            act.pc[act.np++] = fw;
+                        } else if (ct == 3) {   // T-T: act + reversed s
+                            act.pc[act.np++] = bw;
+                        } else {                // H-T: s + act;  H-H: reversed s + act
+                            for (int u = act.np; u > 0; --u) act.pc[u] = act.pc[u - 1];
+                            act.pc[0] = (ct == 1) ? fw : bw;
+                            act.np++;
+                            act.par = (ct == 1) ? s.par : -s.par;
+                        }
+                        act.t = act.t + (s.len - 1) + 1;
+                        cand[q].empty = 1;
+                    }
+                    if (tid == 0) s_con[ci] = act;
+                    maxm -= 2;
+                }
+            }
+            __syncthreads();
+            for (int ci = 0; ci < 3; ++ci) {
+                const ActSeg& a = s_con[ci];
+                const int L = a.empty ? 0 : a.t + 1;
+                do_contour(a.pc, a.np, L, L, (double)(L + 1), a.par);
+            }
+        }
+    }
+    const double m = fabs(total) / (PI * rho * rho);
+    if constexpr (VJP) {
+        const double base = (total >= 0.0 ? 1.0 : -1.0) / (PI * rho * rho);
+        const double gm = A.g_mag[src];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            double v = block_sum(gacc[d], s_d) * base;
+            if (d == 2) v -= 2.0 * m / rho;
+            if (tid == 0) A.g_out[src * ND + d] = gm * v;
+        }
+    }
+    if (tid == 0) {
+        A.mag[src] = m;
+        if (A.status) A.status[src] = s_fail ? 0 : 1;
+    }
+}
+
+template <int NL>
+int mag_extended_nl(hipStream_t s, const ExtArgs& a, int mode, bool ld, bool vjp);
+
+#define CAUS_EXT_CASE(NL, MODE, LD, VJP)                                                       \
+    hipLaunchKernelGGL((k_mag_extended<NL, MODE, LD, VJP>), grid, block, 0, s, a);              \
+    break;
+#define CAUS_DEFINE_EXT(NL)                                                                     \
+    template <>                                                                                 \
+    int mag_extended_nl<NL>(hipStream_t s, const ExtArgs& a, int mode, bool ld, bool vjp) {     \
+        const dim3 grid((unsigned)a.S), block(kExtBlock);                                       \
+        switch ((mode == MODE_CF32 ? 4 : 0) + (ld ? 2 : 0) + (vjp ? 1 : 0)) {                   \
+            case 0: CAUS_EXT_CASE(NL, MODE_FP64, false, false)                                  \
+            case 1: CAUS_EXT_CASE(NL, MODE_FP64, false, true)                                   \
+            case 2: CAUS_EXT_CASE(NL, MODE_FP64, true, false)                                   \
+            case 3: CAUS_EXT_CASE(NL, MODE_FP64, true, true)                                    \
+            case 4: CAUS_EXT_CASE(NL, MODE_CF32, false, false)                                  \
+            case 5: CAUS_EXT_CASE(NL, MODE_CF32, false, true)                                   \
+            case 6: CAUS_EXT_CASE(NL, MODE_CF32, true, false)                                   \
+            case 7: CAUS_EXT_CASE(NL, MODE_CF32, true, true)                                    \
+            default:                                                                            \
+                return CAUS_ERR_ARG;                                                            \
+        }                                                                                       \
+        return hipGetLastError() == hipSuccess ? CAUS_OK : CAUS_ERR_LAUNCH;                     \
+    }
+
+}  // namespace caus

@@ -1,0 +1,135 @@
+"""Light curves on the GPU.
+
+``mag_point_source_lightcurve``: point-source magnification of L parameter sets at T
+epochs along rectilinear trajectories w(t) = e^{-i alpha}(u0 + i (t - t0)/tE)
+(src/caustics/trajectory.py:152-155 with no parallax and psi -> alpha), one fused
+HIP kernel per (light curve, epoch).  Its gradient with respect to every parameter
+(t0, u0, tE, alpha and the lens parameters) comes from the fused forward+VJP kernel,
+which re-solves the roots, forms per-epoch implicit-derivative tangents and reduces
+them per light curve in a fixed order (deterministic).
+
+``lightcurve_fwd_vjp`` exposes that fused kernel directly: one root solve gives the
+magnifications and the vector-Jacobian product for a cotangent ``g_mag``.
+"""
+import numpy as np
+import torch
+
+from . import _lib
+from .point_source import PARAM_NAMES, _mode_id
+
+__all__ = ["mag_point_source_lightcurve", "lightcurve_fwd_vjp", "TRAJ_NAMES"]
+
+TRAJ_NAMES = ("t0", "u0", "tE", "alpha")
+
+
+def _call(t, lcp, g_mag, nlenses, itmax, mode, want_status=False):
+    L_ = _lib.lib()
+    T = t.numel()
+    L = lcp.shape[0]
+    mag = torch.empty((L, T), dtype=torch.float64, device=t.device)
+    status = torch.empty((L, T), dtype=torch.uint8, device=t.device) if want_status else None
+    g_params = ws = None
+    wsb = 0
+    if g_mag is not None:
+        g_params = torch.empty_like(lcp)
+        wsb = L_.caus_lightcurve_workspace_bytes(nlenses, T, L)
+        ws = torch.empty(max(wsb // 8, 1), dtype=torch.float64, device=t.device)
+    rc = L_.caus_lightcurve_point_source(_lib.stream_ptr(), nlenses, _lib.ptr(t), T,
+                                         _lib.ptr(lcp), L, _lib.ptr(g_mag), _lib.ptr(mag),
+                                         _lib.ptr(g_params), _lib.ptr(ws), wsb, _lib.ptr(status),
+                                         itmax, mode)
+    _lib.check(rc, "caus_lightcurve_point_source")
+    return mag, g_params, status
+
+
+def lightcurve_fwd_vjp(t, lc_params, g_mag, nlenses=2, roots_itmax=2500, mode="fp64",
+                       return_status=False):
+    """Fused forward + VJP.  t (T,), lc_params (L, 4+P) rows (t0, u0, tE, alpha, lens
+    params...), g_mag (L, T) -> (mag (L, T), g_params (L, 4+P)[, status (L, T)])."""
+    t = torch.as_tensor(t, device="cuda", dtype=torch.float64).contiguous()
+    lcp = torch.as_tensor(lc_params, device="cuda", dtype=torch.float64).contiguous()
+    g = None if g_mag is None else torch.as_tensor(g_mag, device="cuda",
+                                                   dtype=torch.float64).contiguous()
+    mag, gp, st = _call(t, lcp, g, nlenses, int(roots_itmax), _lib.MODES[mode], return_status)
+    return (mag, gp, st) if return_status else (mag, gp)
+
+
+class _LightCurve(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, nlenses, itmax, mode, t, lcp):
+        mag, _, _ = _call(t, lcp, None, nlenses, itmax, mode)
+        ctx.save_for_backward(t, lcp)
+        ctx.meta = (nlenses, itmax, mode)
+        return mag
+
+    @staticmethod
+    def backward(ctx, g):
+        t, lcp = ctx.saved_tensors
+        nlenses, itmax, mode = ctx.meta
+        _, gp, _ = _call(t, lcp, g.contiguous().to(torch.float64), nlenses, itmax, mode)
+        return None, None, None, None, gp
+
+
+def mag_point_source_lightcurve(t, t0, u0, tE, alpha, nlenses=2, roots_itmax=2500,
+                                roots_compensated=False, mode=None, **params):
+    """Point-source light curves: returns (L, T) magnifications, L = broadcast length
+    of the parameters (each a float or a (L,) tensor).  Differentiable in every
+    parameter (not in t)."""
+    names = TRAJ_NAMES + PARAM_NAMES[nlenses]
+    vals = dict(t0=t0, u0=u0, tE=tE, alpha=alpha, **params)
+    cols = [torch.as_tensor(np.asarray(vals[n]) if not isinstance(vals[n], torch.Tensor)
+                            else vals[n], device="cuda").to(torch.float64) for n in names]
+    L = max([c.numel() for c in cols])
+    lcp = torch.stack([c.reshape(-1).expand(L) for c in cols], -1).contiguous()
+    t = torch.as_tensor(t, device="cuda", dtype=torch.float64).contiguous().reshape(-1)
+    return _LightCurve.apply(nlenses, int(roots_itmax), _mode_id(roots_compensated, mode), t,
+                             lcp)
+
+
+def hexadecapole_gate(w_points, rho, nlenses=2, u1=0.0, roots_itmax=2500, **params):
+    """Per point: hexadecapole magnification, the size of its quadrupole + hexadecapole
+    terms, and whether mag() accepts it (src/caustics/lightcurve.py:200-227;
+    multipole.py:208-244).  Returns (mu_hex, delta_mu, use_hex) tensors."""
+    from .extended_source import _col
+
+    w = w_points if isinstance(w_points, torch.Tensor) else torch.as_tensor(np.asarray(w_points))
+    shape = tuple(w.shape)
+    w = w.to(device="cuda", dtype=torch.complex128).reshape(-1).contiguous()
+    P = w.numel()
+    rho_t = _col(rho, P)
+    names = PARAM_NAMES[nlenses]
+    prm = torch.stack([_col(params[k], P) for k in names], -1).contiguous() if names else None
+    mu = torch.empty(P, dtype=torch.float64, device="cuda")
+    dmu = torch.empty(P, dtype=torch.float64, device="cuda")
+    use = torch.empty(P, dtype=torch.uint8, device="cuda")
+    rc = _lib.lib().caus_hexadecapole_gate(_lib.stream_ptr(), nlenses, _lib.ptr(w), P,
+                                           _lib.ptr(prm), len(names), _lib.ptr(rho_t), 1,
+                                           float(u1), _lib.ptr(mu), _lib.ptr(dmu),
+                                           _lib.ptr(use), None, int(roots_itmax))
+    _lib.check(rc, "caus_hexadecapole_gate")
+    return mu.reshape(shape), dmu.reshape(shape), use.reshape(shape).bool()
+
+
+def mag(w_points, rho, nlenses=2, npts_limb=200, limb_darkening=False, u1=0.0, npts_ld=100,
+        roots_itmax=2500, roots_compensated=False, mode=None, **params):
+    """Extended-source magnification at source positions w_points (reference:
+    src/caustics/lightcurve.py:88-254): the hexadecapole value where the gate accepts
+    it, the full contour integral elsewhere.  As in the reference the multipole terms
+    of the gate use no limb darkening; three-lens systems always use the contour."""
+    from .extended_source import _col, mag_extended_source
+
+    w = w_points if isinstance(w_points, torch.Tensor) else torch.as_tensor(np.asarray(w_points))
+    shape = tuple(w.shape)
+    w = w.to(device="cuda", dtype=torch.complex128).reshape(-1).contiguous()
+    P = w.numel()
+    mu, _, use = hexadecapole_gate(w, rho, nlenses, 0.0, roots_itmax, **params)
+    out = mu.clone()
+    idx = torch.nonzero(~use).reshape(-1)
+    if idx.numel():
+        names = PARAM_NAMES[nlenses]
+        sub = {k: _col(params[k], P)[idx] for k in names}
+        out[idx] = mag_extended_source(w[idx], _col(rho, P)[idx], nlenses=nlenses,
+                                       npts_limb=npts_limb, limb_darkening=limb_darkening,
+                                       u1=_col(u1, P)[idx], npts_ld=npts_ld,
+                                       roots_itmax=roots_itmax, mode=mode, **sub)
+    return out.reshape(shape)

@@ -81,6 +81,95 @@ int caus_gpu_ehrlich_aberth(void* stream, void** buffers, const char* opaque, si
 int caus_roots_vjp(void* stream, const double* coeffs, const double* roots, const double* g_roots,
                    double* g_coeffs, int64_t N, int32_t deg);
 
+/* Point-source magnification, fused (reference: mag_point_source,
+ * src/caustics/point_source.py:1762-1830, with _images_point_source :1652-1709,
+ * lens_eq_det_jac :1558-1580 and the coefficients :17-86 / :89-1479).
+ *   w        (N,) complex source positions (frame of the reference API)
+ *   params   rows of (s, q) for nlenses=2, (s, q, q3, r3, psi) for nlenses=3, unused for 1;
+ *            row i at params + i*params_stride (params_stride = 0 broadcasts one row)
+ *   mag      (N,) output;  status (N,) uint8 or NULL (1 = every root converged)
+ * mode CAUS_MODE_*; initial estimates are always CAUS_INIT_COMPLEX (order-free sum). */
+int caus_mag_point_source(void* stream, int32_t nlenses, const double* w, int64_t N,
+                          const double* params, int64_t params_stride, double* mag,
+                          uint8_t* status, int32_t itmax, int32_t mode);
+
+/* Forward + VJP in one pass (one root solve per evaluation).  For cotangents g_mag (N,):
+ *   g_w      (N,) complex or NULL:  g * (dmag/dRe w + i dmag/dIm w)
+ *   g_params (N, P) or NULL:        g * dmag/dparam_k per evaluation (P = 2 or 5)
+ * The derivative is the implicit one of the lens equation at each real image
+ * (equal to the reference's polynomial-root JVP, ehrlich_aberth_primitive.py:254-324,
+ * chained through the coefficients); the image filter is piecewise constant. */
+int caus_mag_point_source_vjp(void* stream, int32_t nlenses, const double* w, int64_t N,
+                              const double* params, int64_t params_stride, const double* g_mag,
+                              double* mag, double* g_w, double* g_params, uint8_t* status,
+                              int32_t itmax, int32_t mode);
+
+/* Point-source light curves along rectilinear trajectories w(t) = e^{-i alpha}(u0 + i (t-t0)/tE)
+ * (src/caustics/trajectory.py:152-155 without parallax), L light curves x T epochs.
+ *   t          (T,) epochs;  lc_params (L, 4+P) rows (t0, u0, tE, alpha, lens params...)
+ *   mag        (L, T) output;  status (L, T) uint8 or NULL
+ *   g_mag      (L, T) cotangents or NULL (forward only)
+ *   g_params   (L, 4+P) output: sum over epochs of g_mag * dmag/dparam, reduced in a
+ *              fixed order (deterministic); needs workspace of
+ *              caus_lightcurve_workspace_bytes(nlenses, T, L) bytes.  L <= 65535. */
+int64_t caus_lightcurve_workspace_bytes(int32_t nlenses, int64_t T, int64_t L);
+int caus_lightcurve_point_source(void* stream, int32_t nlenses, const double* t, int64_t T,
+                                 const double* lc_params, int64_t L, const double* g_mag,
+                                 double* mag, double* g_params, double* workspace,
+                                 int64_t workspace_bytes, uint8_t* status, int32_t itmax,
+                                 int32_t mode);
+
+/* Hexadecapole magnification and contour gate of mag() (src/caustics/lightcurve.py:168-227):
+ * _mag_hexadecapole (src/caustics/multipole.py:208-244) with linear limb darkening u1
+ * (mag() itself passes u1 = 0, lightcurve.py:215), _caustics_proximity_test
+ * (lightcurve.py:23-75) and, for q < 0.01, _planetary_caustic_test (lightcurve.py:78-85).
+ *   rho      source radius per point at rho + i*rho_stride
+ *   mu_hex   (P,) hexadecapole magnification; delta_mu (P,) or NULL: sum |dq|+|dh|
+ *   use_hex  (P,) uint8: 1 where the hexadecapole value is accepted, 0 where the full
+ *            contour integral is needed (always 0 for nlenses=3; |w| > 2 rho for one lens) */
+int caus_hexadecapole_gate(void* stream, int32_t nlenses, const double* w, int64_t P,
+                           const double* params, int64_t params_stride, const double* rho,
+                           int64_t rho_stride, double u1, double* mu_hex, double* delta_mu,
+                           uint8_t* use_hex, uint8_t* status, int32_t itmax);
+
+/* Extended-source magnification by contour integration, one workgroup per source
+ * (reference: mag_extended_source, src/caustics/extended_source.py:730-901, with
+ * _images_of_source_limb :56-153, segments and stitching :156-727 and the integrals of
+ * src/caustics/integrate.py:18-121).
+ *   w0 (S,) complex source centres; rho (S,); params rows as for caus_mag_point_source
+ *   npts_limb (S,) int32 per-source limb resolution (the reference's npts_limb) or NULL to
+ *            use npts_max for every source; each value must lie in [4, npts_max]
+ *            (out-of-range values are clamped and flagged in status)
+ *   npts_max  largest npts_limb, <= 4096 (sets the workspace size)
+ *   limb_darkening 0: uniform source (trapezoid Green integral); 1: linear limb darkening
+ *            with coefficient u1[i*u1_stride] and npts_ld Gauss-Legendre points per line
+ *            integral (P, Q of Dominik 1998), npts_ld in [2, 256]
+ *   mag (S,) output; status (S,) uint8 or NULL (0: a root solve did not converge or
+ *            npts_limb was out of range)
+ *   workspace: caus_extended_workspace_bytes(nlenses, S, npts_max) bytes of device memory
+ * mode CAUS_MODE_FP64 or CAUS_MODE_CF32. */
+int64_t caus_extended_workspace_bytes(int32_t nlenses, int64_t S, int32_t npts_max);
+int caus_mag_extended_source(void* stream, int32_t nlenses, const double* w0, int64_t S,
+                             const double* rho, const double* params, int64_t params_stride,
+                             const int32_t* npts_limb, int32_t npts_max, int32_t limb_darkening,
+                             const double* u1, int64_t u1_stride, int32_t npts_ld, double* mag,
+                             uint8_t* status, void* workspace, int64_t workspace_bytes,
+                             int32_t itmax, int32_t mode);
+
+/* Forward + VJP of caus_mag_extended_source in one pass.  For cotangents g_mag (S,) it
+ * writes g_params (S, 4+P): g * dmag/d(Re w0, Im w0, rho, u1, lens params...), the
+ * derivative of the discretised integral as the reference's autodiff takes it (limb
+ * angles, image rows and segment structure fixed; every contour point differentiated
+ * implicitly through the lens equation; Gauss-Legendre integrals with their moving end
+ * and split points). */
+int caus_mag_extended_source_vjp(void* stream, int32_t nlenses, const double* w0, int64_t S,
+                                 const double* rho, const double* params, int64_t params_stride,
+                                 const int32_t* npts_limb, int32_t npts_max,
+                                 int32_t limb_darkening, const double* u1, int64_t u1_stride,
+                                 int32_t npts_ld, const double* g_mag, double* mag,
+                                 double* g_params, uint8_t* status, void* workspace,
+                                 int64_t workspace_bytes, int32_t itmax, int32_t mode);
+
 #ifdef __cplusplus
 }
 #endif

@@ -152,10 +152,18 @@ def lens_eq(z, nlenses=2, **lp):
     return z - e1 / (zb - a) - e2 / (zb + a) - (1.0 - e1 - e2) / (zb - np.conj(r3))
 
 
-def lens_eq_det_jac(z, nlenses=2, **lp):
+def lens_eq_det_jac(z, nlenses=2, reference_single_lens=False, **lp):
+    """det J = 1 - |sum_j e_j/(conj z - conj r_j)^2|^2 (point_source.py:1558-1580).
+
+    For one lens the reference returns 1 - 1/|z|^2 (point_source.py:1562-1563), which
+    is not the Jacobian determinant 1 - 1/|z|^4 of w = z - 1/conj(z); the sign agrees,
+    the magnification does not (the Paczynski closed form pins the correct one).
+    ``reference_single_lens=True`` restates the reference expression."""
     zb = np.conj(z)
     if nlenses == 1:
-        return 1.0 - 1.0 / np.abs(zb**2)
+        if reference_single_lens:
+            return 1.0 - 1.0 / np.abs(zb**2)
+        return 1.0 - 1.0 / np.abs(zb**2) ** 2
     if nlenses == 2:
         a, e1 = lp["a"], lp["e1"]
         return 1.0 - np.abs(e1 / (zb - a) ** 2 + (1.0 - e1) / (zb + a) ** 2) ** 2
@@ -371,3 +379,50 @@ def mag_point_source_and_grad(w, nlenses=2, itmax=2500, **params):
     mag = (mag_i * mask).sum(-1)
     grad = np.real((dmag_i * mask[..., None]).sum(-2))
     return mag, grad
+
+
+# ----------------------------------------------------------------------------
+# critical curves and caustics (point_source.py:1482-1534, 1582-1649)
+# ----------------------------------------------------------------------------
+def poly_coeffs_critical(phi, nlenses, **lp):
+    """Highest-first coefficients of the critical-curve polynomial
+    sum_j e_j / (z - r_j)^2 = exp(-i phi), times prod_j (z - r_j)^2 (notebook cells 19/22)."""
+    x = np.exp(-1j * np.asarray(phi, float))
+    if nlenses == 2:
+        a, e1 = lp["a"], lp["e1"]
+        p = [x, 0 * x, -2 * a**2 * x - 1.0, (-4 * a * e1 + 2 * a) * np.ones_like(x),
+             a**4 * x - a**2]
+        return np.stack(p, -1)
+    a, r3, e1, e2 = lp["a"], lp["r3"], lp["e1"], lp["e2"]
+    p0 = x
+    p1 = -2 * x * r3
+    p2 = -2 * a**2 * x - 1 + x * r3**2
+    p3 = 4 * a**2 * x * r3 - 2 * a * e1 + 2 * a * e2 + 2 * e1 * r3 + 2 * e2 * r3
+    p4 = (a**4 * x - 3 * a**2 * e1 - 3 * a**2 * e2 + 2 * a**2 - 2 * a**2 * x * r3**2
+          + 4 * a * e1 * r3 - 4 * a * e2 * r3 - e1 * r3**2 - e2 * r3**2)
+    p5 = (-2 * a**4 * x * r3 + 2 * a**2 * e1 * r3 + 2 * a**2 * e2 * r3 - 2 * a * e1 * r3**2
+          + 2 * a * e2 * r3**2)
+    p6 = (a**4 * e1 + a**4 * e2 - a**4 + a**4 * x * r3**2 - a**2 * e1 * r3**2
+          - a**2 * e2 * r3**2)
+    return np.stack([p0, p1, p2, p3, p4, p5, p6 * np.ones_like(x)], -1)
+
+
+def critical_and_caustic_curves(npts=200, nlenses=2, **params):
+    """(critical curves, caustics), each (n_roots, npts), in the centre-of-mass frame."""
+    from .extended import match_points
+
+    phi = np.linspace(-np.pi, np.pi, npts)
+    if nlenses == 1:
+        return np.exp(-1j * phi), np.zeros(npts, complex)
+    lp, x_cm = lens_params(nlenses, **params)
+    c = poly_coeffs_critical(phi, nlenses, **lp)
+    z = ea.poly_roots(c)
+    carry = z[0]
+    out = []
+    for k in range(npts):
+        idx = match_points(carry, z[k])
+        carry = z[k][idx]
+        out.append(carry)
+    zc = np.array(out).T
+    za = lens_eq(zc, nlenses, **lp)
+    return zc - x_cm, za - x_cm

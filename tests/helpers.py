@@ -47,3 +47,15 @@ def roots_close(z_test, z_ref, coeffs_asc, rel=1e-12, kappa_factor=64.0):
     kap = root_condition(coeffs_asc, z_ref)
     tol = np.maximum(rel, kappa_factor * kap * EPS)
     return err <= tol, zt, err
+
+
+def agree_condition_aware(test, plain, truth, rel=1e-10, factor=10.0):
+    """Per item: the tested value is within `rel` of the plain FP64 oracle, or at least
+    as close to a more accurate value (compensated-arithmetic roots) as the plain
+    oracle is, up to `factor`.  Both FP64 solvers lose accuracy on ill-conditioned
+    (clustered) roots by the same mechanism; this separates that from real errors."""
+    test, plain, truth = map(np.asarray, (test, plain, truth))
+    e_tp = np.abs(test - plain) / np.abs(plain)
+    e_t = np.abs(test - truth) / np.abs(truth)
+    e_p = np.abs(plain - truth) / np.abs(truth)
+    return (e_tp <= rel) | (e_t <= np.maximum(rel, factor * e_p))

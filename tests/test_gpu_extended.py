@@ -1,0 +1,175 @@
+"""Extended-source contour kernels vs the oracle (oracle/extended.py restating
+extended_source.py / integrate.py) and vs closed forms.
+
+Bars: within 1e-10 relative of the oracle for nearly every source (a discrete choice
+near a threshold may differ: those stay within the discretisation error, 1e-3), and
+within the reference tests' own rtol 1e-3 of the single-lens closed forms."""
+import numpy as np
+import pytest
+from scipy.integrate import quad
+
+from caustics_amd.extended_source import mag_extended_source
+from oracle import extended as E
+from oracle import lens
+
+pytestmark = pytest.mark.gpu
+
+
+def single_lens_uniform(u, rho):
+    """Uniform disk: (1/pi rho^2) int_0^{u+rho} (r^2+2)/sqrt(r^2+4) * arc(r) dr."""
+    def arc(r):
+        if r < abs(rho - u):
+            return 2 * np.pi if u < rho else 0.0
+        c = (r * r + u * u - rho * rho) / (2 * r * u)
+        return 2 * np.arccos(np.clip(c, -1, 1))
+    f = lambda r: (r * r + 2) / np.sqrt(r * r + 4) * arc(r)
+    v, _ = quad(f, 0, u + rho, points=[abs(u - rho)], limit=200, epsabs=1e-13, epsrel=1e-12)
+    return v / (np.pi * rho * rho)
+
+
+def compare(m, mo, name):
+    e = np.abs(m - mo) / np.abs(mo)
+    print(f"{name}: n={len(m)} max rel err {e.max():.3e}, frac<=1e-10 {(e <= 1e-10).mean():.3f}")
+    assert (e <= 1e-10).mean() >= 0.9
+    assert e.max() < 1e-3
+
+
+@pytest.mark.parametrize("rho", [1.0, 0.1, 0.01])
+def test_single_lens_uniform(gpu, rho):
+    w = np.linspace(0.0, 3.0 * rho, 11) + 1e-12
+    m = mag_extended_source(w, rho, nlenses=1, npts_limb=150).cpu().numpy()
+    mo = np.array([E.mag_extended_source(x, rho, nlenses=1, npts_limb=150) for x in w])
+    compare(m, mo, f"single uniform rho={rho}")
+    exact = np.array([single_lens_uniform(abs(x), rho) for x in w])
+    np.testing.assert_allclose(m, exact, rtol=1e-3)
+
+
+def test_single_lens_ld_reduces_to_uniform(gpu):
+    rho = 0.1
+    w = np.linspace(0.0, 3.0 * rho, 11) + 1e-12
+    mu = mag_extended_source(w, rho, nlenses=1, npts_limb=150).cpu().numpy()
+    ml = mag_extended_source(w, rho, nlenses=1, npts_limb=150, limb_darkening=True, u1=0.0,
+                             npts_ld=50).cpu().numpy()
+    np.testing.assert_allclose(ml, mu, rtol=1e-3)
+    mo = np.array([E.mag_extended_source(x, rho, nlenses=1, npts_limb=150, limb_darkening=True,
+                                         u1=0.7, npts_ld=100) for x in w])
+    m7 = mag_extended_source(w, rho, nlenses=1, npts_limb=150, limb_darkening=True, u1=0.7,
+                             npts_ld=100).cpu().numpy()
+    compare(m7, mo, "single LD u1=0.7")
+
+
+def caustic_points(rho, nlenses=2, npts=12, seed=42, **params):
+    _, ca = lens.critical_and_caustic_curves(npts=npts, nlenses=nlenses, **params)
+    ca = ca.reshape(-1)
+    g = np.random.default_rng(seed)
+    return ca + g.uniform(0, 2 * rho, ca.shape) * np.exp(1j * g.uniform(-np.pi, np.pi, ca.shape))
+
+
+@pytest.mark.parametrize("rho", [1e-1, 1e-2, 1e-3])
+def test_binary_uniform_near_caustics(gpu, rho):
+    w = caustic_points(rho, s=0.9, q=0.2)
+    m = mag_extended_source(w, rho, nlenses=2, npts_limb=300, s=0.9, q=0.2).cpu().numpy()
+    mo = np.array([E.mag_extended_source(x, rho, nlenses=2, npts_limb=300, s=0.9, q=0.2)
+                   for x in w])
+    compare(m, mo, f"binary uniform rho={rho}")
+
+
+def test_binary_ld_near_caustics(gpu):
+    rho = 1e-2
+    w = caustic_points(rho, npts=6, s=0.9, q=0.2)
+    kw = dict(nlenses=2, npts_limb=300, limb_darkening=True, u1=0.7, npts_ld=100, s=0.9, q=0.2)
+    m = mag_extended_source(w, rho, **kw).cpu().numpy()
+    mo = np.array([E.mag_extended_source(x, rho, **kw) for x in w])
+    compare(m, mo, "binary LD")
+
+
+def test_triple_near_caustics(gpu):
+    prm = dict(s=1.2, q=0.3, q3=0.01, r3=0.8, psi=1.0)
+    rho = 1e-2
+    w = caustic_points(rho, nlenses=3, npts=6, **prm)
+    m = mag_extended_source(w, rho, nlenses=3, npts_limb=200, **prm).cpu().numpy()
+    mo = np.array([E.mag_extended_source(x, rho, nlenses=3, npts_limb=200, **prm) for x in w])
+    compare(m, mo, "triple uniform")
+
+
+def test_ragged_batch_matches_individual(gpu):
+    rho = 2e-2
+    w = caustic_points(rho, npts=5, s=1.1, q=0.3)
+    npts = np.array([256, 300, 512, 150, 400] * (len(w) // 5 + 1))[:len(w)]
+    m = mag_extended_source(w, rho, nlenses=2, npts_limb=npts, s=1.1, q=0.3).cpu().numpy()
+    for i in range(len(w)):
+        mi = mag_extended_source(w[i:i + 1], rho, nlenses=2, npts_limb=int(npts[i]), s=1.1,
+                                 q=0.3).cpu().numpy()
+        assert m[i] == mi[0]
+
+
+def _fd(fun, x0, h):
+    return (fun(x0 + h) - fun(x0 - h)) / (2 * h)
+
+
+@pytest.mark.parametrize("ld", [False, True])
+def test_extended_vjp_vs_finite_differences(gpu, ld):
+    # the reference's gradient test (tests/test_extended_source.py:294-331): d mag / d
+    # (s, q, rho, u1) at a caustic point, rtol 1e-3 against finite differences
+    from caustics_amd.extended_source import extended_fwd_vjp
+
+    s, q, rho, u1 = 0.9, 0.2, 1e-2, 0.7
+    w = caustic_points(rho, npts=6, s=s, q=q)[::3]
+    kw = dict(nlenses=2, npts_limb=300, limb_darkening=ld, npts_ld=100)
+    mag, g, st = extended_fwd_vjp(w, rho, np.ones(len(w)), u1=u1, s=s, q=q, **kw)
+    g = g.cpu().numpy()
+    f = lambda **o: mag_extended_source(w, o.get("rho", rho), u1=o.get("u1", u1),
+                                        s=o.get("s", s), q=o.get("q", q), **kw).cpu().numpy()
+    h = 1e-7
+    checks = {"rho": 2, "s": 4, "q": 5}
+    if ld:
+        checks["u1"] = 3
+    for name, col in checks.items():
+        base = dict(rho=rho, s=s, q=q, u1=u1)
+        bp, bm = dict(base), dict(base)
+        bp[name] += h
+        bm[name] -= h
+        fd = (f(**bp) - f(**bm)) / (2 * h)
+        err = np.abs(g[:, col] - fd) / np.maximum(np.abs(fd), 1e-3 * mag.cpu().numpy())
+        print(name, "max rel err vs FD", err.max())
+        assert np.median(err) < 1e-3
+        assert np.mean(err < 1e-2) >= 0.8
+    # source position
+    fw = lambda d: mag_extended_source(w + d, rho, u1=u1, s=s, q=q, **kw).cpu().numpy()
+    fdx = (fw(h) - fw(-h)) / (2 * h)
+    fdy = (fw(1j * h) - fw(-1j * h)) / (2 * h)
+    ex = np.abs(g[:, 0] - fdx) / np.maximum(np.abs(fdx), 1e-3 * mag.cpu().numpy())
+    ey = np.abs(g[:, 1] - fdy) / np.maximum(np.abs(fdy), 1e-3 * mag.cpu().numpy())
+    assert np.median(ex) < 1e-3 and np.median(ey) < 1e-3
+
+
+def test_extended_autograd_matches_fused(gpu):
+    import torch
+
+    rho = 2e-2
+    w = caustic_points(rho, npts=4, s=1.1, q=0.3)
+    wt = torch.as_tensor(w, device="cuda").requires_grad_(True)
+    st = torch.tensor(1.1, device="cuda", dtype=torch.float64, requires_grad=True)
+    m = mag_extended_source(wt, rho, nlenses=2, npts_limb=200, s=st, q=0.3)
+    m.sum().backward()
+    from caustics_amd.extended_source import extended_fwd_vjp
+
+    _, g, _ = extended_fwd_vjp(w, rho, np.ones(len(w)), nlenses=2, npts_limb=200, s=1.1, q=0.3)
+    g = g.cpu().numpy()
+    np.testing.assert_allclose(wt.grad.cpu().numpy(), g[:, 0] + 1j * g[:, 1], rtol=1e-12)
+    np.testing.assert_allclose(st.grad.item(), g[:, 4].sum(), rtol=1e-12)
+
+
+def test_mag_lightcurve_gate_and_contours(gpu):
+    from caustics_amd import mag
+    from oracle import multipole
+
+    s, q, rho = 0.9, 0.2, 1e-2
+    w = caustic_points(rho, npts=10, s=s, q=q)
+    w = np.concatenate([w, w + 0.1])
+    m = mag(w, rho, nlenses=2, npts_limb=200, s=s, q=q).cpu().numpy()
+    mu, use = multipole.hexadecapole_gate(w, rho, nlenses=2, s=s, q=q)
+    mo = np.where(use, mu, 0.0)
+    for i in np.nonzero(~use)[0]:
+        mo[i] = E.mag_extended_source(w[i], rho, nlenses=2, npts_limb=200, s=s, q=q)
+    compare(m, mo, "mag() with gate")

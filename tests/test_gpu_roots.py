@@ -60,7 +60,8 @@ def test_reference_fixture_real_coefficients(gpu):
         for idx in [(3, 1), (4, 0)]:
             np.testing.assert_allclose(np.polyval(c[idx], r[idx]), 0, atol=1e-10)
         ro = ea.poly_roots(c, compensated=comp)
-        np.testing.assert_allclose(r, ro, rtol=1e-12, atol=1e-14)
+        ok, _, _ = roots_close(r.reshape(-1, 5), ro.reshape(-1, 5), c.reshape(-1, 6)[:, ::-1])
+        assert ok.all()
 
 
 def test_batching_shapes_equivalent(gpu):
@@ -119,7 +120,10 @@ def test_descriptor_entry_point(gpu):
 
 
 def test_roots_vjp_matches_implicit_derivative_and_fd(gpu):
-    c = lens_polys(2, 64, seed=21)
+    # well-separated roots, so that a 1e-6 coefficient step stays in the linear regime
+    g0 = np.random.default_rng(21)
+    zs = g0.normal(size=(64, 5)) + 1j * g0.normal(size=(64, 5))
+    c = np.array([np.poly(z) for z in zs])[:, ::-1].copy()
     ct = torch.as_tensor(c, device="cuda").requires_grad_(True)
     z = ca.ehrlich_aberth(ct).reshape(64, 5)
     g = np.random.default_rng(1)

@@ -1,0 +1,129 @@
+"""Pins the extended-source oracle (oracle/extended.py).
+
+Known-answer tests from the reference's own suite (tests/test_extended_source.py:186-206
+_split_segment arrays; tests/test_utils.py:32-42 trapz_zero_avoiding), closed forms
+for one lens (uniform disk: 1-D integral of the Paczynski magnification over the
+disk; linear limb darkening: 2-D integral), u1=0 == uniform, invariants (no duplicate
+images on the limb, stitched contours close), and a source-plane quadrature of the
+point-source magnification for binary sources, at the reference's tolerance 1e-3."""
+import numpy as np
+import pytest
+from scipy.integrate import quad
+
+from oracle import extended as E
+from oracle import lens
+
+
+def test_split_segment_known_answer():
+    z = np.array([0.0, 1.0, 1.01, 1.02, 1.03, 0.0, 4.0, 4.01, 0.0, 0.0, 6.0]).astype(complex)
+    par = np.array([-1.0, -1, -1, 1, 1, 1, 1, 1, 1, 1, 1])
+    mask = np.array([0.0, 1, 1, 1, 1, 0, 1, 1, 0, 0, 1])
+    res = E.split_segment(np.stack([z, par, mask]), n_parts=5)[:, 0, :].real
+    np.testing.assert_equal(res[0], [0.0, 1.0, 1.01, 0, 0, 0, 0, 0, 0, 0, 0])
+    np.testing.assert_equal(res[1], [0.0, 0, 0, 1.02, 1.03, 0, 0, 0, 0, 0, 0])
+    np.testing.assert_equal(res[2], [0.0, 0, 0, 0, 0, 0, 4.0, 4.01, 0, 0, 0])
+    np.testing.assert_equal(res[3], [0.0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 6.0])
+    np.testing.assert_equal(res[4], np.zeros(11))
+
+
+def test_trapz_zero_avoiding_known_answer():
+    x = np.linspace(0, 1, 13)
+    y = np.cos(x)
+    ref = E.trapz(y, x)
+    xp = np.append(x, np.zeros(7))
+    yp = np.append(y, np.zeros(7))
+    assert E.trapz_zero_avoiding(yp, xp, 12) == pytest.approx(ref, rel=1e-15)
+
+
+def single_uniform(u, rho):
+    def arc(r):
+        if r < abs(rho - u):
+            return 2 * np.pi if u < rho else 0.0
+        c = (r * r + u * u - rho * rho) / (2 * r * u)
+        return 2 * np.arccos(np.clip(c, -1, 1))
+    f = lambda r: (r * r + 2) / np.sqrt(r * r + 4) * arc(r)
+    v, _ = quad(f, 0, u + rho, points=[abs(u - rho)], limit=200, epsabs=1e-13, epsrel=1e-12)
+    return v / (np.pi * rho * rho)
+
+
+def single_ld(u, rho, u1):
+    """(1/pi rho^2) int int_disk A(|w|) I(|w - u|/rho) dA with the normalised linear profile."""
+    def inner(r):
+        # angles phi where the point r e^{i phi} lies in the disk around (u, 0)
+        if r < abs(rho - u):
+            lim = np.pi if u < rho else 0.0
+        else:
+            c = (r * r + u * u - rho * rho) / (2 * r * u)
+            lim = np.arccos(np.clip(c, -1, 1))
+        if lim == 0.0:
+            return 0.0
+        def prof(phi):
+            d2 = r * r + u * u - 2 * r * u * np.cos(phi)
+            x = max(1 - d2 / (rho * rho), 0.0)
+            return 3 / (3 - u1) * (1 - u1 * (1 - np.sqrt(x)))
+        v, _ = quad(prof, -lim, lim, limit=200, epsabs=1e-12)
+        return v
+    f = lambda r: (r * r + 2) / np.sqrt(r * r + 4) * inner(r)
+    v, _ = quad(f, 0, u + rho, points=[abs(u - rho)], limit=200, epsabs=1e-11, epsrel=1e-10)
+    return v / (np.pi * rho * rho)
+
+
+@pytest.mark.parametrize("rho", [1.0, 0.1, 0.01, 0.001])
+def test_single_lens_uniform_closed_form(rho):
+    w = np.linspace(0.0, 3.0 * rho, 11) + 1e-12
+    m = np.array([E.mag_extended_source(x, rho, nlenses=1, npts_limb=150) for x in w])
+    ex = np.array([single_uniform(abs(x), rho) for x in w])
+    np.testing.assert_allclose(m, ex, rtol=1e-3)
+
+
+@pytest.mark.parametrize("rho", [1.0, 0.1, 0.01])
+def test_single_lens_ld_closed_form_and_u1_zero(rho):
+    w = np.linspace(0.0, 3.0 * rho, 6) + 1e-12
+    m = np.array([E.mag_extended_source(x, rho, nlenses=1, npts_limb=300, limb_darkening=True,
+                                        u1=0.7, npts_ld=100) for x in w])
+    ex = np.array([single_ld(abs(x), rho, 0.7) for x in w])
+    np.testing.assert_allclose(m, ex, rtol=1e-3)
+    m0 = np.array([E.mag_extended_source(x, rho, nlenses=1, npts_limb=150, limb_darkening=True,
+                                         u1=0.0, npts_ld=50) for x in w])
+    mu = np.array([E.mag_extended_source(x, rho, nlenses=1, npts_limb=150) for x in w])
+    np.testing.assert_allclose(m0, mu, rtol=1e-3)
+
+
+def source_plane_quadrature(w0, rho, nr=500, nphi=1000, **params):
+    """Midpoint rule in polar coordinates over the source disk of the point-source
+    magnification (an independent route to the same integral)."""
+    r = (np.arange(nr) + 0.5) / nr * rho
+    phi = (np.arange(nphi) + 0.5) / nphi * 2 * np.pi
+    R, PH = np.meshgrid(r, phi, indexing="ij")
+    w = w0 + R * np.exp(1j * PH)
+    m = lens.mag_point_source(w.ravel(), nlenses=2, **params).reshape(w.shape)
+    return np.sum(m * R) * (rho / nr) * (2 * np.pi / nphi) / (np.pi * rho**2)
+
+
+@pytest.mark.slow
+def test_binary_vs_source_plane_quadrature():
+    s, q = 0.9, 0.2
+    for w0, rho in [(0.3 + 0.25j, 0.05), (-0.6 + 0.1j, 0.1), (0.05 + 0.02j, 0.05)]:
+        m = E.mag_extended_source(w0, rho, nlenses=2, npts_limb=600, s=s, q=q)
+        ref = source_plane_quadrature(w0, rho, s=s, q=q)
+        assert abs(m / ref - 1) < 2e-3, (w0, rho, m, ref)
+
+
+def test_triple_limb_images_unique_and_contours_closed():
+    prm = dict(s=1.2, q=0.3, q3=0.01, r3=0.8, psi=1.0)
+    _, ca = lens.critical_and_caustic_curves(npts=8, nlenses=3, **prm)
+    lp, x_cm = lens.lens_params(3, **prm)
+    for w in ca.reshape(-1)[::3]:
+        rho = 0.02
+        z, mask, par, th = E.images_of_source_limb(w + x_cm, rho, 3, 200, lp)
+        zz = z.reshape(-1)
+        assert len(np.unique(zz)) == len(zz)
+        seg_c, seg_o, all_closed = E.get_segments(z, mask, par, 3)
+        if not all_closed:
+            c, cp = E.contours_from_open_segments(seg_o)
+            for k in range(3):
+                if np.all(c[k] == 0):
+                    continue
+                t = E.last_nonzero(c[k].real)
+                seg = np.stack([c[k], np.ones_like(c[k])])
+                assert E.connection_condition(seg, seg, t, t, 0)
