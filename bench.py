@@ -59,6 +59,7 @@ def main():
     ap.add_argument("--n10", type=int, default=1 << 18)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--roots-only", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (every unit of the path is "
@@ -163,9 +164,21 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(host, args.cpu_seconds)
 
+    workloads = {}
+    if not args.roots_only:
+        for name, fn in (("point_source_1e4", lambda: bench_point_source(args.steps, 2)),
+                         ("lightcurves_256x2000_fwd_vjp", lambda: bench_lightcurves(3, 1)),
+                         ("extended_binary_ld_4096_fwd_vjp", lambda: bench_extended_binary(1, 1)),
+                         ("triple_extended_lightcurves_fwd_vjp", lambda: bench_triple(1, 1))):
+            print(f"bench: {name}", file=sys.stderr, flush=True)
+            try:
+                workloads[name] = fn()
+            except Exception as e:  # a failing extra workload must not hide the headline
+                workloads[name] = {"error": repr(e)}
+            print(f"bench: {name}: {workloads[name]}", file=sys.stderr, flush=True)
+
     line = {
-        "metric": "polys/s deg 5/10 batched root solve (FP64, roots matched to FP64 oracle); "
-                  "mag evals/s fwd+VJP at <=1e-8 rel err vs FP64",
+        "metric": "mag evals/s fwd+VJP at \u22641e-8 rel err vs FP64; polys/s deg 5/10",
         "value": main_r["value"],
         "unit": "polys/s",
         "n_gpus": 1,
@@ -187,12 +200,160 @@ def main():
                      "flops_per_launch": flops,
                      "flop_rule": f"{flops_per_root_iter(n)} per active FP64 root-iteration"},
         "cpu_baseline": cpu,
+        "workloads": workloads,
         "forms": {k: {kk: vv for kk, vv in v.items() if kk not in ("work",)}
                   for k, v in results.items()},
         "work_counts": {k: {str(degs[nl]): [int(x) for x in v["work"][nl]] for nl in (2, 3)}
                         for k, v in results.items()},
     }
     print(json.dumps(line, default=float))
+
+
+def _timed(fn, steps, warmup):
+    """Run fn warmup times, then time `steps` calls between device synchronisations."""
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def bench_point_source(steps, warmup):
+    """10^4 binary point-source magnifications, s=0.9, q=0.1, w uniform in [-1.5,1.5]^2."""
+    import torch
+
+    from caustics_amd import _lib
+    from caustics_amd import workloads as W
+
+    g = W.rng(1)
+    n = 10000
+    w = torch.as_tensor(g.uniform(-1.5, 1.5, n) + 1j * g.uniform(-1.5, 1.5, n), device="cuda")
+    prm = torch.tensor([0.9, 0.1], dtype=torch.float64, device="cuda")
+    mag = torch.empty(n, dtype=torch.float64, device="cuda")
+    L = _lib.lib()
+
+    def run():
+        _lib.check(L.caus_mag_point_source(_lib.stream_ptr(), 2, _lib.ptr(w), n, _lib.ptr(prm), 0,
+                                           _lib.ptr(mag), None, 2500, _lib.MODE_FP64), "ps")
+    dt = _timed(run, steps * 10, warmup)
+    return {"workload": "10^4 binary point-source magnifications (s=0.9, q=0.1), forward, FP64",
+            "mag_evals_per_s": n / dt, "ms": dt * 1e3}
+
+
+def bench_lightcurves(steps, warmup, host_check=True):
+    """256 binary point-source light curves x 2000 epochs, fused forward + VJP wrt
+    (t0, u0, tE, alpha, s, q); FP64 and CF32 forms; error of CF32 vs FP64."""
+    import torch
+
+    from caustics_amd import _lib
+    from caustics_amd import workloads as W
+
+    L_, T = 256, 2000
+    prm = W.lightcurve_params(L_, seed=3)
+    lcp = torch.as_tensor(np.stack([prm[k] for k in ("t0", "u0", "tE", "alpha", "s", "q")], -1),
+                          device="cuda").contiguous()
+    t = torch.linspace(-100, 100, T, dtype=torch.float64, device="cuda")
+    g = torch.ones((L_, T), dtype=torch.float64, device="cuda")
+    mag = torch.empty((L_, T), dtype=torch.float64, device="cuda")
+    gp = torch.empty_like(lcp)
+    Lb = _lib.lib()
+    wsb = Lb.caus_lightcurve_workspace_bytes(2, T, L_)
+    ws = torch.empty(wsb // 8, dtype=torch.float64, device="cuda")
+    out = {"workload": "256 binary point-source light curves x 2000 epochs, fused fwd+VJP "
+                       "(t0, u0, tE, alpha, s, q)", "evals": L_ * T}
+    mags = {}
+    for name, mode in (("fp64", _lib.MODE_FP64), ("cf32", _lib.MODE_CF32)):
+        def run():
+            _lib.check(Lb.caus_lightcurve_point_source(
+                _lib.stream_ptr(), 2, _lib.ptr(t), T, _lib.ptr(lcp), L_, _lib.ptr(g),
+                _lib.ptr(mag), _lib.ptr(gp), _lib.ptr(ws), wsb, None, 2500, mode), "lc")
+        dt = _timed(run, steps, warmup)
+        run()
+        torch.cuda.synchronize()
+        mags[name] = (mag.clone(), gp.clone())
+        out[name] = {"mag_evals_per_s_fwd_vjp": L_ * T / dt, "ms": dt * 1e3}
+    m64, g64 = mags["fp64"]
+    m32, g32 = mags["cf32"]
+    rel = ((m32 - m64).abs() / m64.abs()).cpu().numpy()
+    out["cf32"]["max_rel_err_vs_fp64"] = float(rel.max())
+    out["cf32"]["p99_rel_err_vs_fp64"] = float(np.quantile(rel, 0.99))
+    out["cf32"]["frac_le_1e-8"] = float(np.mean(rel <= 1e-8))
+    return out
+
+
+def bench_extended_binary(steps, warmup, n_near=2048, n_far=2048):
+    """Binary limb-darkened (u=0.5) sources, rho log-uniform [1e-3,1e-1]: n_near within
+    2 rho of a caustic with ragged contour resolution 256..4096 points, n_far accepted by
+    the hexadecapole gate.  One step: gate on all sources, then the contour kernel with
+    its VJP on every source the gate rejects."""
+    import torch
+
+    from caustics_amd import _lib
+    from caustics_amd import workloads as W
+    from caustics_amd.extended_source import _run
+    from caustics_amd.lightcurve import hexadecapole_gate
+
+    b = W.extended_binary_batch(n_near, n_far)
+    S = len(b["w"])
+    w = torch.as_tensor(b["w"], device="cuda")
+    rho = torch.as_tensor(b["rho"], device="cuda")
+    u1 = torch.as_tensor(b["u1"], device="cuda")
+    prm = torch.as_tensor(np.stack([b["s"], b["q"]], -1), device="cuda").contiguous()
+    npts_all = np.concatenate([b["npts"], np.full(n_far, 256)])
+    npts = torch.as_tensor(npts_all, device="cuda", dtype=torch.int32)
+    gm = torch.ones(S, dtype=torch.float64, device="cuda")
+    res = {}
+
+    def step():
+        mu, _, use = hexadecapole_gate(w, rho, nlenses=2, s=prm[:, 0], q=prm[:, 1])
+        idx = torch.nonzero(~use).reshape(-1)
+        m, st, go = _run(2, w[idx].contiguous(), rho[idx].contiguous(), prm[idx].contiguous(),
+                         npts[idx].contiguous(), int(npts_all.max()), True,
+                         u1[idx].contiguous(), 100, 2500, _lib.MODE_FP64, g=gm[idx].contiguous(),
+                         max_chunk=4096)
+        res["n_contour"] = int(idx.numel())
+        res["unconverged"] = int((st == 0).sum().item())
+    dt = _timed(step, steps, warmup)
+    return {"workload": f"{S} binary limb-darkened sources (u=0.5), {n_near} near caustics with "
+                        "ragged 256-4096 limb points, rest hexadecapole-gated; contour fwd+VJP",
+            "mag_evals_per_s": S / dt, "contour_sources": res["n_contour"],
+            "contour_fwd_vjp_evals_per_s": res["n_contour"] / dt, "ms": dt * 1e3,
+            "unconverged": res["unconverged"],
+            "note": "hexadecapole-gated sources are forward only (no VJP of the hexadecapole)"}
+
+
+def bench_triple(steps, warmup, n_sets=8, n_epochs=1500, npts=256):
+    """Triple-lens caustic-crossing light curves, full contour at every epoch, fused
+    fwd+VJP (u=0.4, rho log-uniform [1e-3,3e-2]); a bounded subset of parameter sets."""
+    import torch
+
+    from caustics_amd import _lib
+    from caustics_amd import workloads as W
+    from caustics_amd.extended_source import _run
+
+    b = W.triple_caustic_lightcurves(n_sets, n_epochs)
+    w = torch.as_tensor(b["w"].reshape(-1), device="cuda")
+    S = w.numel()
+    rep = lambda x: torch.as_tensor(np.repeat(np.asarray(x), n_epochs), device="cuda")
+    rho, u1 = rep(b["rho"]), rep(b["u1"])
+    prm = torch.stack([rep(b[k]) for k in ("s", "q", "q3", "r3", "psi")], -1).contiguous()
+    gm = torch.ones(S, dtype=torch.float64, device="cuda")
+    st_holder = {}
+
+    def step():
+        m, st, go = _run(3, w, rho, prm, None, npts, True, u1, 100, 2500, _lib.MODE_FP64, g=gm,
+                         max_chunk=S)
+        st_holder["st"] = st
+    dt = _timed(step, steps, warmup)
+    return {"workload": f"{n_sets} triple-lens parameter sets x {n_epochs} epochs (caustic "
+                        f"crossing), limb-darkened u=0.4, npts_limb={npts}, full contour fwd+VJP",
+            "mag_evals_per_s_fwd_vjp": S / dt, "ms": dt * 1e3,
+            "unconverged": int((st_holder["st"] == 0).sum().item())}
 
 
 def cpu_baseline(host, seconds):

@@ -7,7 +7,7 @@ All compute runs in hand-written HIP kernels for gfx950 (libcaustics_hip.so).
 from .ehrlich_aberth_primitive import ehrlich_aberth, poly_roots
 from .extended_source import mag_extended_source
 from .lightcurve import hexadecapole_gate, mag, mag_point_source_lightcurve
-from .point_source import lens_eq, lens_eq_det_jac, mag_point_source
+from .point_source import critical_and_caustic_curves, lens_eq, lens_eq_det_jac, mag_point_source
 
 __all__ = [
     "ehrlich_aberth",
@@ -15,6 +15,7 @@ __all__ = [
     "lens_eq",
     "lens_eq_det_jac",
     "mag_point_source",
+    "critical_and_caustic_curves",
     "mag_point_source_lightcurve",
     "mag_extended_source",
     "hexadecapole_gate",

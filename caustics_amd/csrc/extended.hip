@@ -79,3 +79,53 @@ extern "C" int caus_mag_extended_source_vjp(void* stream, int32_t nlenses, const
             return mag_extended_nl<3>(s, a, mode, limb_darkening != 0, vjp);
     }
 }
+
+namespace {
+
+// Greedy continuation of root sets along a sequence (match_points chained as in
+// critical_and_caustic_curves, point_source.py:1628-1640): one lane per chain.
+template <int DEG>
+__global__ void k_match_chain(const cd* __restrict__ pts, cd* __restrict__ out, int64_t npts,
+                              int64_t nchains) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nchains) return;
+    const cd* p = pts + c * npts * DEG;
+    cd* o = out + c * npts * DEG;
+    cd carry[DEG];
+#pragma unroll
+    for (int r = 0; r < DEG; ++r) carry[r] = p[r];
+    for (int64_t k = 0; k < npts; ++k) {
+        cd b[DEG];
+#pragma unroll
+        for (int r = 0; r < DEG; ++r) b[r] = p[k * DEG + r];
+        const uint64_t perm = greedy_match<DEG>(carry, b);
+        permute_pt<DEG>(b, perm);
+#pragma unroll
+        for (int r = 0; r < DEG; ++r) {
+            carry[r] = b[r];
+            o[k * DEG + r] = b[r];
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" int caus_match_chain(void* stream, const double* pts, double* out, int64_t npts,
+                                int32_t n, int64_t nchains) {
+    if (npts < 0 || nchains < 0 || n < 1) return CAUS_ERR_ARG;
+    if (npts == 0 || nchains == 0) return CAUS_OK;
+    if (!pts || !out || pts == out) return CAUS_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    const dim3 g((unsigned)((nchains + 63) / 64)), b(64);
+    const cd* p = reinterpret_cast<const cd*>(pts);
+    cd* o = reinterpret_cast<cd*>(out);
+    switch (n) {
+        case 2: hipLaunchKernelGGL(k_match_chain<2>, g, b, 0, s, p, o, npts, nchains); break;
+        case 4: hipLaunchKernelGGL(k_match_chain<4>, g, b, 0, s, p, o, npts, nchains); break;
+        case 5: hipLaunchKernelGGL(k_match_chain<5>, g, b, 0, s, p, o, npts, nchains); break;
+        case 6: hipLaunchKernelGGL(k_match_chain<6>, g, b, 0, s, p, o, npts, nchains); break;
+        case 10: hipLaunchKernelGGL(k_match_chain<10>, g, b, 0, s, p, o, npts, nchains); break;
+        default: return CAUS_ERR_ARG;
+    }
+    return hipGetLastError() == hipSuccess ? CAUS_OK : CAUS_ERR_LAUNCH;
+}

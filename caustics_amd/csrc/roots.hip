@@ -10,6 +10,24 @@
 
 using namespace caus;
 
+namespace caus {
+// Workgroups of the persistent root kernels: 4 per compute unit of the current device.
+int roots_grid_limit() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 1024;
+    static int cached[64] = {0};
+    if (dev < 0 || dev >= 64) return 1024;
+    if (cached[dev] == 0) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            cus <= 0)
+            cus = 256;
+        cached[dev] = 4 * cus;
+    }
+    return cached[dev];
+}
+}  // namespace caus
+
 
 extern "C" int caus_roots(void* stream, const double* coeffs, const double* roots_init,
                           double* roots, int32_t* stats, uint8_t* status, int64_t N, int32_t deg,

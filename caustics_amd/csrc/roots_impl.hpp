@@ -44,12 +44,96 @@ __global__ __launch_bounds__(kRootsBlock) void k_roots(const cd* __restrict__ co
     }
 }
 
+// Initial estimates for every polynomial (init_est.h:57-102), written to `roots`.
+template <int DEG, bool REFCOMPAT>
+__global__ __launch_bounds__(kRootsBlock) void k_roots_init(const cd* __restrict__ coeffs,
+                                                            cd* __restrict__ roots, int64_t N) {
+    const int64_t i = (int64_t)blockIdx.x * kRootsBlock + threadIdx.x;
+    if (i >= N) return;
+    double al[DEG + 1];
+#pragma unroll
+    for (int k = 0; k <= DEG; ++k) al[k] = cabs(coeffs[i * (DEG + 1) + k]);
+    cd z[DEG];
+    init_est<DEG, REFCOMPAT>(al, z);
+#pragma unroll
+    for (int k = 0; k < DEG; ++k) roots[i * DEG + k] = z[k];
+}
+
+// Persistent solve with lane refill: each lane walks the polynomials i, i + L, i + 2L,
+// ... (L = lanes in the grid) and starts its next polynomial as soon as the current one
+// has converged, so a wave no longer waits for its slowest polynomial at every step.
+// Initial estimates come from `init` (k_roots_init output or the caller's roots_init).
+template <int DEG, int MODE, bool REFCOMPAT>
+__global__ __launch_bounds__(kRootsBlock) void k_roots_persistent(
+    const cd* __restrict__ coeffs, const cd* init, cd* roots, int32_t* __restrict__ stats,
+    uint8_t* __restrict__ status, int64_t N, int itmax) {
+    const int64_t L = (int64_t)gridDim.x * kRootsBlock;
+    int64_t i = (int64_t)blockIdx.x * kRootsBlock + threadIdx.x;
+    cd p[DEG + 1], z[DEG];
+    EAState<DEG, MODE, REFCOMPAT> S;
+    SolveStats st{0, 0, 0, 0};
+    bool active = i < N;
+    if (active) {
+#pragma unroll
+        for (int k = 0; k <= DEG; ++k) p[k] = coeffs[i * (DEG + 1) + k];
+#pragma unroll
+        for (int k = 0; k < DEG; ++k) z[k] = init[i * DEG + k];
+        S.start(p, z, true);
+    }
+    while (__ballot(active) != 0ull) {
+        if (active) {
+            const bool done = S.sweep(p, z, st);
+            if (done || st.sweeps >= itmax) {
+#pragma unroll
+                for (int k = 0; k < DEG; ++k) roots[i * DEG + k] = z[k];
+                if (status) status[i] = done ? 1 : 0;
+                if (stats) {
+                    stats[4 * i + 0] = st.sweeps;
+                    stats[4 * i + 1] = st.it64;
+                    stats[4 * i + 2] = st.itcomp;
+                    stats[4 * i + 3] = 0;
+                }
+                i += L;
+                active = i < N;
+                if (active) {
+#pragma unroll
+                    for (int k = 0; k <= DEG; ++k) p[k] = coeffs[i * (DEG + 1) + k];
+#pragma unroll
+                    for (int k = 0; k < DEG; ++k) z[k] = init[i * DEG + k];
+                    S.start(p, z, true);
+                    st = SolveStats{0, 0, 0, 0};
+                }
+            }
+        }
+    }
+}
+
+int roots_grid_limit();
+
 template <int DEG, int MODE, bool RC>
 int launch_roots(hipStream_t s, const cd* c, const cd* ri, cd* r, int32_t* st, uint8_t* ok,
                  int64_t N, int itmax) {
     const int64_t blocks = (N + kRootsBlock - 1) / kRootsBlock;
-    hipLaunchKernelGGL((k_roots<DEG, MODE, RC>), dim3((unsigned)blocks), dim3(kRootsBlock), 0, s,
-                       c, ri, r, st, ok, N, itmax);
+    if (MODE == MODE_CF32) {
+        hipLaunchKernelGGL((k_roots<DEG, MODE, RC>), dim3((unsigned)blocks), dim3(kRootsBlock), 0,
+                           s, c, ri, r, st, ok, N, itmax);
+        return hipGetLastError() == hipSuccess ? CAUS_OK : CAUS_ERR_LAUNCH;
+    }
+    if (itmax == 0) {
+        // no sweep at all: the initial estimates are the result
+        hipLaunchKernelGGL((k_roots<DEG, MODE, RC>), dim3((unsigned)blocks), dim3(kRootsBlock), 0,
+                           s, c, ri, r, st, ok, N, itmax);
+        return hipGetLastError() == hipSuccess ? CAUS_OK : CAUS_ERR_LAUNCH;
+    }
+    const cd* init = ri;
+    if (!ri) {
+        hipLaunchKernelGGL((k_roots_init<DEG, RC>), dim3((unsigned)blocks), dim3(kRootsBlock), 0, s,
+                           c, r, N);
+        init = r;
+    }
+    const int64_t pb = blocks < roots_grid_limit() ? blocks : roots_grid_limit();
+    hipLaunchKernelGGL((k_roots_persistent<DEG, MODE, RC>), dim3((unsigned)pb), dim3(kRootsBlock),
+                       0, s, c, init, r, st, ok, N, itmax);
     return hipGetLastError() == hipSuccess ? CAUS_OK : CAUS_ERR_LAUNCH;
 }
 

@@ -307,27 +307,27 @@ __device__ __forceinline__ bool ref_step(const cd (&p)[DEG + 1], const double (&
     }
 }
 
-// Solve one polynomial held in registers.  Returns true when all roots converged
-// (at the compensated level for MODE_FP64_COMP).  z holds the initial estimates on
-// entry when custom_init is set.
+// Iteration state of one polynomial (registers of one lane).
 template <int DEG, int MODE, bool REFCOMPAT>
-__device__ __forceinline__ bool solve_fp64(const cd (&p)[DEG + 1], cd (&z)[DEG], int itmax,
-                                           bool custom_init, SolveStats& st) {
-    double al[DEG + 1];
+struct EAState {
+    double al[DEG + 1];  // alpha' weights of the stopping test
+    unsigned cx, cy;     // per-root convergence: plain stage, compensated stage
+    int nrev;
+
+    // alpha_i = |p_i| (ehrlich_aberth.h:77-82), initial estimates unless supplied,
+    // alpha_i *= 1 + (1+2 sqrt 2) i (:96-99)
+    __device__ __forceinline__ void start(const cd (&p)[DEG + 1], cd (&z)[DEG], bool custom_init) {
 #pragma unroll
-    for (int i = 0; i <= DEG; ++i) al[i] = cabs(p[i]);
-    if (!custom_init) init_est<DEG, REFCOMPAT>(al, z);
+        for (int i = 0; i <= DEG; ++i) al[i] = cabs(p[i]);
+        if (!custom_init) init_est<DEG, REFCOMPAT>(al, z);
 #pragma unroll
-    for (int i = 0; i <= DEG; ++i) al[i] = al[i] * fma(ALPHA_W, (double)i, 1.0);
-    constexpr unsigned ALL = (1u << DEG) - 1u;
-    unsigned cx = 0, cy = 0;
-    int nrev = 0;
-    st.sweeps = 0;
-    st.it64 = 0;
-    st.itcomp = 0;
-    st.it32 = 0;
-    bool done = false;
-    for (int it = 0; it < itmax; ++it) {
+        for (int i = 0; i <= DEG; ++i) al[i] = al[i] * fma(ALPHA_W, (double)i, 1.0);
+        cx = cy = 0;
+        nrev = 0;
+    }
+    // one Gauss-Seidel sweep over the roots (ehrlich_aberth.h:102-143); returns true
+    // when every root has converged
+    __device__ __forceinline__ bool sweep(const cd (&p)[DEG + 1], cd (&z)[DEG], SolveStats& st) {
 #pragma unroll
         for (int j = 0; j < DEG; ++j) {
             if (!((cx >> j) & 1u)) {
@@ -339,10 +339,26 @@ __device__ __forceinline__ bool solve_fp64(const cd (&p)[DEG + 1], cd (&z)[DEG],
                 if (comp_step<DEG, REFCOMPAT>(p, z, j)) cy |= 1u << j;
             }
         }
-        st.sweeps = it + 1;
-        done = (MODE == MODE_FP64_COMP) ? (cy == ALL) : (cx == ALL);
-        if (done) break;
+        ++st.sweeps;
+        constexpr unsigned ALL = (1u << DEG) - 1u;
+        return (MODE == MODE_FP64_COMP) ? (cy == ALL) : (cx == ALL);
     }
+};
+
+// Solve one polynomial held in registers.  Returns true when all roots converged
+// (at the compensated level for MODE_FP64_COMP).  z holds the initial estimates on
+// entry when custom_init is set.
+template <int DEG, int MODE, bool REFCOMPAT>
+__device__ __forceinline__ bool solve_fp64(const cd (&p)[DEG + 1], cd (&z)[DEG], int itmax,
+                                           bool custom_init, SolveStats& st) {
+    EAState<DEG, MODE, REFCOMPAT> S;
+    S.start(p, z, custom_init);
+    st.sweeps = 0;
+    st.it64 = 0;
+    st.itcomp = 0;
+    st.it32 = 0;
+    bool done = false;
+    for (int it = 0; it < itmax && !done; ++it) done = S.sweep(p, z, st);
     return done;
 }
 

@@ -15,7 +15,13 @@ import torch
 
 from . import _lib
 
-__all__ = ["mag_point_source", "lens_eq", "lens_eq_det_jac", "PARAM_NAMES"]
+__all__ = [
+    "mag_point_source",
+    "lens_eq",
+    "lens_eq_det_jac",
+    "critical_and_caustic_curves",
+    "PARAM_NAMES",
+]
 
 PARAM_NAMES = {1: (), 2: ("s", "q"), 3: ("s", "q", "q3", "r3", "psi")}
 
@@ -145,3 +151,67 @@ def lens_eq_det_jac(z, nlenses=2, **params):
         r_c = torch.conj(r) if isinstance(r, torch.Tensor) else r
         K = K + e / (zb - r_c) ** 2
     return 1.0 - torch.abs(K) ** 2
+
+
+def _critical_coeffs(phi, nlenses, a, e1, e2=None, r3=None):
+    """Highest-first coefficients of prod_j (z - r_j)^2 exp(-i phi) - sum_j e_j
+    prod_{k!=j} (z - r_k)^2, the critical-curve polynomial (point_source.py:1482-1534)."""
+    x = torch.exp(-1j * phi)
+    one = torch.ones_like(x)
+    if nlenses == 2:
+        return torch.stack([x, 0 * x, -2 * a**2 * x - 1.0, (-4 * a * e1 + 2 * a) * one,
+                            a**4 * x - a**2], -1)
+    p = [x, -2 * x * r3, -2 * a**2 * x - 1 + x * r3**2,
+         4 * a**2 * x * r3 - 2 * a * e1 + 2 * a * e2 + 2 * e1 * r3 + 2 * e2 * r3,
+         (a**4 * x - 3 * a**2 * e1 - 3 * a**2 * e2 + 2 * a**2 - 2 * a**2 * x * r3**2
+          + 4 * a * e1 * r3 - 4 * a * e2 * r3 - e1 * r3**2 - e2 * r3**2),
+         (-2 * a**4 * x * r3 + 2 * a**2 * e1 * r3 + 2 * a**2 * e2 * r3 - 2 * a * e1 * r3**2
+          + 2 * a * e2 * r3**2),
+         (a**4 * e1 + a**4 * e2 - a**4 + a**4 * x * r3**2 - a**2 * e1 * r3**2
+          - a**2 * e2 * r3**2) * one]
+    return torch.stack(p, -1)
+
+
+def critical_and_caustic_curves(npts=200, nlenses=2, **params):
+    """Critical curves and caustics, each (n_curves, npts) complex, in the frame of the
+    reference API (point_source.py:1582-1649): roots of the critical-curve polynomial
+    on a grid of phi (GPU solver), continued along phi with match_points (HIP kernel),
+    mapped through the lens equation and shifted by x_cm.  Parameters may also be
+    arrays of length B, giving (B, n_curves, npts)."""
+    from .ehrlich_aberth_primitive import poly_roots
+
+    phi = torch.linspace(-np.pi, np.pi, npts, dtype=torch.float64, device="cuda")
+    if nlenses == 1:
+        return torch.exp(-1j * phi), torch.zeros(npts, dtype=torch.complex128, device="cuda")
+    names = PARAM_NAMES[nlenses]
+    vals = [torch.as_tensor(np.asarray(params[k]) if not isinstance(params[k], torch.Tensor)
+                            else params[k], device="cuda").to(torch.float64) for k in names]
+    batch = vals[0].dim() > 0
+    vals = [v.reshape(-1, 1) for v in vals]
+    s, q = vals[0], vals[1]
+    a = 0.5 * s
+    x_cm = a * (1 - q) / (1 + q)
+    if nlenses == 2:
+        e1 = 1 / (1 + q)
+        coeffs = _critical_coeffs(phi[None, :], 2, a, e1)
+        lf = dict(a=a, e1=e1)
+    else:
+        q3, r3m, psi = vals[2], vals[3], vals[4]
+        e1 = q / (1 + q + q3)
+        e2 = q * e1
+        r3 = r3m * torch.exp(1j * psi)
+        coeffs = _critical_coeffs(phi[None, :], 3, a, e1, e2, r3)
+        lf = dict(a=a, e1=e1, e2=e2, r3=r3)
+    z = poly_roots(coeffs).contiguous()            # (B, npts, n)
+    B, _, n = z.shape
+    zc = torch.empty_like(z)
+    rc = _lib.lib().caus_match_chain(_lib.stream_ptr(), _lib.ptr(z), _lib.ptr(zc), npts, n, B)
+    _lib.check(rc, "caus_match_chain")
+    zc = zc.transpose(1, 2)                        # (B, n, npts)
+    lfb = {k: (v[:, :, None] if isinstance(v, torch.Tensor) else v) for k, v in lf.items()}
+    za = lens_eq(zc, nlenses=nlenses, **lfb)
+    zc = zc - x_cm[:, :, None]
+    za = za - x_cm[:, :, None]
+    if not batch:
+        return zc[0], za[0]
+    return zc, za

@@ -133,3 +133,61 @@ def trajectory(t, t0, u0, tE, alpha):
     ca, sa = np.cos(alpha)[:, None], np.sin(alpha)[:, None]
     u0 = np.asarray(u0)[:, None]
     return (u0 * ca + tau * sa) + 1j * (-u0 * sa + tau * ca)
+
+
+# ---------------------------------------------------------------------------
+# extended-source workloads (GPU-side generation: caustic points come from the
+# GPU critical-curve solver, far sources are rejection-sampled with the gate)
+# ---------------------------------------------------------------------------
+def extended_binary_batch(n_near, n_far, seed=4, npts_choices=(256, 512, 1024, 2048, 4096),
+                          u1=0.5):
+    """Binary limb-darkened sources: n_near placed within 2 rho of a caustic point of
+    their own lens (as in the reference's tests/test_extended_source.py:82-97), n_far
+    uniform in |w| <= 2 and accepted by the hexadecapole gate; rho log-uniform
+    [1e-3, 1e-1]; s, q as lens_configs.  Returns a dict of numpy arrays."""
+    from .lightcurve import hexadecapole_gate
+    from .point_source import critical_and_caustic_curves
+
+    g = rng(seed)
+    p = lens_configs(g, n_near + n_far, 2)
+    rho = loguniform(g, 1e-3, 1e-1, n_near + n_far)
+    s, q = p["s"], p["q"]
+    _, ca = critical_and_caustic_curves(npts=64, nlenses=2, s=s[:n_near], q=q[:n_near])
+    ca = ca.reshape(n_near, -1).cpu().numpy()
+    pick = ca[np.arange(n_near), g.integers(0, ca.shape[1], n_near)]
+    r = g.uniform(0, 2 * rho[:n_near])
+    w_near = pick + r * np.exp(1j * g.uniform(-np.pi, np.pi, n_near))
+    w_far = np.zeros(n_far, complex)
+    todo = np.arange(n_far)
+    while len(todo):
+        cand = disk(g, len(todo))
+        _, _, use = hexadecapole_gate(cand, rho[n_near + todo], nlenses=2, s=s[n_near + todo],
+                                      q=q[n_near + todo])
+        ok = use.cpu().numpy()
+        w_far[todo[ok]] = cand[ok]
+        todo = todo[~ok]
+    npts = g.choice(np.asarray(npts_choices), n_near)
+    return dict(w=np.concatenate([w_near, w_far]), rho=rho, s=s, q=q, u1=np.full(len(rho), u1),
+                npts=npts, n_near=n_near)
+
+
+def triple_caustic_lightcurves(n_sets, n_epochs, seed=5, u1=0.4):
+    """Triple-lens light curves crossing a caustic: per parameter set s, q, q3, r3, psi as
+    lens_configs, rho log-uniform [1e-3, 3e-2], tE = 20 d, and a rectilinear trajectory
+    (angle alpha random) through a caustic point at t = 0; epochs t in [-2 tE, 2 tE]."""
+    from .point_source import critical_and_caustic_curves
+
+    g = rng(seed)
+    p = lens_configs(g, n_sets, 3)
+    rho = loguniform(g, 1e-3, 3e-2, n_sets)
+    _, ca = critical_and_caustic_curves(npts=64, nlenses=3, **p)
+    ca = ca.reshape(n_sets, -1).cpu().numpy()
+    c = ca[np.arange(n_sets), g.integers(0, ca.shape[1], n_sets)]
+    alpha = g.uniform(0, 2 * np.pi, n_sets)
+    tE = np.full(n_sets, 20.0)
+    rot = c * np.exp(1j * alpha)            # = u0 + i tau_c
+    u0, tau_c = rot.real, rot.imag
+    t0 = -tau_c * tE
+    t = np.linspace(-2 * 20.0, 2 * 20.0, n_epochs)
+    w = trajectory(t, t0, u0, tE, alpha)    # (n_sets, n_epochs)
+    return dict(w=w, rho=rho, u1=np.full(n_sets, u1), t=t, **p)

@@ -170,6 +170,14 @@ int caus_mag_extended_source_vjp(void* stream, int32_t nlenses, const double* w0
                                  double* g_params, uint8_t* status, void* workspace,
                                  int64_t workspace_bytes, int32_t itmax, int32_t mode);
 
+/* Greedy continuation of root sets along a sequence: for each of nchains chains of npts
+ * sets of n points (pts (nchains, npts, n) complex), out[k] = pts[k] permuted so that
+ * each row continues the previous set by the reference's match_points rule
+ * (src/caustics/utils.py:15-40, chained as in critical_and_caustic_curves,
+ * src/caustics/point_source.py:1628-1640).  n in {2, 4, 5, 6, 10}; pts != out. */
+int caus_match_chain(void* stream, const double* pts, double* out, int64_t npts, int32_t n,
+                     int64_t nchains);
+
 #ifdef __cplusplus
 }
 #endif

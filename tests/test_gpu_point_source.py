@@ -162,3 +162,25 @@ def test_lightcurve_fwd_and_vjp(gpu):
         e = np.abs(fd - gp[:, k]) / np.maximum(np.abs(fd), 1e-3 * np.abs(g * mo).sum(1))
         print(key, np.max(e))
         assert np.median(e) < 1e-5
+
+
+@pytest.mark.parametrize("nlenses", [2, 3])
+def test_critical_and_caustic_curves(gpu, nlenses):
+    from caustics_amd import critical_and_caustic_curves
+
+    prm = dict(s=0.9, q=0.2) if nlenses == 2 else dict(s=1.2, q=0.3, q3=0.01, r3=0.8, psi=1.0)
+    zc, za = critical_and_caustic_curves(npts=200, nlenses=nlenses, **prm)
+    zco, zao = lens.critical_and_caustic_curves(npts=200, nlenses=nlenses, **prm)
+    zc, za = zc.cpu().numpy(), za.cpu().numpy()
+    assert zc.shape == zco.shape
+    # same curves as sets of rows (the first phi's root order may differ)
+    from helpers import match_rows
+
+    perm_rows = match_rows(zco[:, :1].T, zc[:, :1].T).T
+    order = [int(np.argmin(np.abs(zc[:, 0] - v))) for v in perm_rows[:, 0]]
+    np.testing.assert_allclose(zc[order], zco, atol=1e-9)
+    np.testing.assert_allclose(za[order], zao, atol=1e-8)
+    # critical curve points satisfy det J = 0 (lens frame)
+    lp, x_cm = lens.lens_params(nlenses, **prm)
+    det = lens.lens_eq_det_jac(zc + x_cm, nlenses, **lp)
+    assert np.max(np.abs(det)) < 1e-9
