@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libcaustics_hip.so")
 ARCH = os.environ.get("CAUS_OFFLOAD_ARCH", "gfx950")
 
 HIPFLAGS = [
-    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=fast",
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=fast-honor-pragmas",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", CSRC, "-I", os.path.join(ROOT, "include"),
 ]

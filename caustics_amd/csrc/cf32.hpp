@@ -21,11 +21,13 @@ namespace caus {
 constexpr float EA_EPS32 = 5.9604644775390625e-08f;  // 2^-24
 
 __device__ __forceinline__ void two_sum_f(float a, float b, float& s, float& e) {
+#pragma clang fp contract(off)
     s = a + b;
     float t = s - a;
     e = (a - (s - t)) + (b - t);
 }
 __device__ __forceinline__ void two_prod_f(float a, float b, float& p, float& e) {
+#pragma clang fp contract(off)
     p = a * b;
     e = fmaf(a, b, -p);
 }
@@ -113,6 +115,7 @@ __device__ __forceinline__ bool solve_cf32(const cd (&p)[DEG + 1], cd (&z)[DEG],
     // ---- stage B: one compensated FP32 sweep, result in double-float
 #pragma unroll
     for (int j = 0; j < DEG; ++j) {
+#pragma clang fp contract(off)
         ++st.itcomp;
         cf x = zf[j];
         cf h = ph[DEG], e = pl[DEG], hd = cf{0.f, 0.f};

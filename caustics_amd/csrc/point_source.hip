@@ -60,7 +60,8 @@ extern "C" int caus_mag_point_source_vjp(void* stream, int32_t nlenses, const do
 
 extern "C" int64_t caus_lightcurve_workspace_bytes(int32_t nlenses, int64_t T, int64_t L) {
     if (nlenses < 1 || nlenses > 3 || T < 0 || L < 0) return -1;
-    const int64_t nchunk = (T + kPsBlock - 1) / kPsBlock;
+    const int64_t per_block = (int64_t)kPsBlock * lc_epochs_per_lane(T, L);
+    const int64_t nchunk = (T + per_block - 1) / per_block;
     return nchunk * L * (4 + np_for(nlenses)) * (int64_t)sizeof(double);
 }
 
@@ -95,7 +96,8 @@ extern "C" int caus_lightcurve_point_source(void* stream, int32_t nlenses, const
     }
     if (rc != CAUS_OK || !vjp) return rc;
     const int nq = 4 + np_for(nlenses);
-    const int nchunk = (int)((T + kPsBlock - 1) / kPsBlock);
+    const int64_t per_block = (int64_t)kPsBlock * lc_epochs_per_lane(T, L);
+    const int nchunk = (int)((T + per_block - 1) / per_block);
     const int64_t n = L * nq;
     hipLaunchKernelGGL(k_lightcurve_finalize, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
                        workspace, g_params, L, nchunk, nq);

@@ -21,17 +21,18 @@ constexpr int ps_degree() {
     return NL * NL + 1;
 }
 
+// warm = true: z holds initial estimates (the roots of a nearby source position)
 template <int NL, int MODE>
 __device__ __forceinline__ bool solve_lens(cd wl, const LensFrame<NL>& lf, cd (&z)[ps_degree<NL>()],
-                                           int itmax) {
+                                           int itmax, bool warm = false) {
     constexpr int DEG = ps_degree<NL>();
     cd p[DEG + 1];
     lens_poly<NL>(wl, lf, p);
     SolveStats st;
     if constexpr (MODE == MODE_CF32)
-        return solve_cf32<DEG>(p, z, itmax, false, st);
+        return solve_cf32<DEG>(p, z, itmax, warm, st);
     else
-        return solve_fp64<DEG, MODE, false>(p, z, itmax, false, st);
+        return solve_fp64<DEG, MODE, false>(p, z, itmax, warm, st);
 }
 
 // Value and tangents of the magnification at lens-frame source position wl.
@@ -41,7 +42,8 @@ __device__ __forceinline__ double mag_with_tangents(cd wl, const LensFrame<NL>& 
                                                     const cd (&dw)[ND > 0 ? ND : 1],
                                                     const LensTangent<NL> (&lt)[ND > 0 ? ND : 1],
                                                     double (&dm)[ND > 0 ? ND : 1], int itmax,
-                                                    bool& ok) {
+                                                    bool& ok, cd* zio = nullptr,
+                                                    bool warm = false) {
 #pragma unroll
     for (int d = 0; d < ND; ++d) dm[d] = 0.0;
     double m = 0.0;
@@ -64,7 +66,15 @@ __device__ __forceinline__ double mag_with_tangents(cd wl, const LensFrame<NL>& 
     } else {
         constexpr int DEG = ps_degree<NL>();
         cd z[DEG];
-        ok = solve_lens<NL, MODE>(wl, lf, z, itmax);
+        if (warm) {
+#pragma unroll
+            for (int j = 0; j < DEG; ++j) z[j] = zio[j];
+        }
+        ok = solve_lens<NL, MODE>(wl, lf, z, itmax, warm);
+        if (zio) {
+#pragma unroll
+            for (int j = 0; j < DEG; ++j) zio[j] = z[j];
+        }
 #pragma unroll
         for (int j = 0; j < DEG; ++j) {
             ImageTerms<NL> it = image_terms<NL>(z[j], lf);
@@ -156,19 +166,34 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// epochs handled by one lane (consecutive, each warm-started from the previous one's
+// roots): enough lanes to fill the device at two waves per SIMD, at most 8 per lane
+__host__ __device__ inline int lc_epochs_per_lane(int64_t T, int64_t L) {
+    const int64_t e = (T * L + 131071) / 131072;
+    return (int)(e < 1 ? 1 : (e > 8 ? 8 : e));
+}
+
 template <int NL, int MODE, bool VJP>
 __global__ __launch_bounds__(kPsBlock) void k_lightcurve_point_source(
     const double* __restrict__ t, int64_t T, const double* __restrict__ lcp,
     const double* __restrict__ g_mag, double* __restrict__ mag, double* __restrict__ ws,
-    uint8_t* __restrict__ status, int itmax) {
+    uint8_t* __restrict__ status, int itmax, int epl) {
     constexpr int NP = NL == 1 ? 0 : n_user_params<NL>();
     constexpr int NQ = 4 + NP;
     constexpr int ND = VJP ? NQ : 0;
+    constexpr int DEG = NL == 1 ? 2 : ps_degree<NL>();
     __shared__ double red[kPsBlock / 64][NQ];
     const int64_t l = blockIdx.y;
-    const int64_t ti = (int64_t)blockIdx.x * kPsBlock + threadIdx.x;
-    const bool active = ti < T;
     const double* prm = lcp + l * NQ;
+    double gsum[NQ];
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) gsum[k] = 0.0;
+    cd zw[DEG];
+    bool warm = false;
+    const int64_t ti0 = ((int64_t)blockIdx.x * kPsBlock + threadIdx.x) * epl;
+    for (int ep = 0; ep < epl; ++ep) {
+    const int64_t ti = ti0 + ep;
+    const bool active = ti < T;
     const double t0 = prm[0], u0 = prm[1], tE = prm[2], alpha = prm[3];
     LensFrame<NL> lf;
     double x_cm = 0.0;
@@ -214,17 +239,23 @@ __global__ __launch_bounds__(kPsBlock) void k_lightcurve_point_source(
     }
     bool ok = true;
     double m = 0.0;
-    if (active) m = mag_with_tangents<NL, MODE, ND>(w + cd{x_cm, 0.0}, lf, dw, lt, dm, itmax, ok);
     if (active) {
+        m = mag_with_tangents<NL, MODE, ND>(w + cd{x_cm, 0.0}, lf, dw, lt, dm, itmax, ok, zw, warm);
+        warm = ok;
         mag[l * T + ti] = m;
         if (status) status[l * T + ti] = ok ? 1 : 0;
+        if constexpr (VJP) {
+            const double g = g_mag[l * T + ti];
+#pragma unroll
+            for (int k = 0; k < NQ; ++k) gsum[k] += g * dm[k];
+        }
+    }
     }
     if constexpr (VJP) {
-        const double g = active ? g_mag[l * T + ti] : 0.0;
         const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
         for (int k = 0; k < NQ; ++k) {
-            const double v = wave_sum(active ? g * dm[k] : 0.0);
+            const double v = wave_sum(gsum[k]);
             if (lane == 0) red[wv][k] = v;
         }
         __syncthreads();
@@ -305,31 +336,33 @@ int lightcurve_point_source_nl(hipStream_t s, const double* t, int64_t T, const 
                                        const double* lcp, int64_t L, const double* g_mag,        \
                                        double* mag, double* ws, uint8_t* status, int itmax,      \
                                        int mode, bool vjp) {                                     \
-        const dim3 grid((unsigned)((T + kPsBlock - 1) / kPsBlock), (unsigned)L), block(kPsBlock); \
+        const int epl = lc_epochs_per_lane(T, L);                                                \
+        const int64_t per_block = (int64_t)kPsBlock * epl;                                       \
+        const dim3 grid((unsigned)((T + per_block - 1) / per_block), (unsigned)L), block(kPsBlock); \
         switch (mode * 2 + (vjp ? 1 : 0)) {                                                      \
             case 0:                                                                              \
                 hipLaunchKernelGGL((k_lightcurve_point_source<NL, MODE_FP64, false>), grid,      \
-                                   block, 0, s, t, T, lcp, g_mag, mag, ws, status, itmax);       \
+                                   block, 0, s, t, T, lcp, g_mag, mag, ws, status, itmax, epl);       \
                 break;                                                                           \
             case 1:                                                                              \
                 hipLaunchKernelGGL((k_lightcurve_point_source<NL, MODE_FP64, true>), grid,       \
-                                   block, 0, s, t, T, lcp, g_mag, mag, ws, status, itmax);       \
+                                   block, 0, s, t, T, lcp, g_mag, mag, ws, status, itmax, epl);       \
                 break;                                                                           \
             case 2:                                                                              \
                 hipLaunchKernelGGL((k_lightcurve_point_source<NL, MODE_FP64_COMP, false>), grid, \
-                                   block, 0, s, t, T, lcp, g_mag, mag, ws, status, itmax);       \
+                                   block, 0, s, t, T, lcp, g_mag, mag, ws, status, itmax, epl);       \
                 break;                                                                           \
             case 3:                                                                              \
                 hipLaunchKernelGGL((k_lightcurve_point_source<NL, MODE_FP64_COMP, true>), grid,  \
-                                   block, 0, s, t, T, lcp, g_mag, mag, ws, status, itmax);       \
+                                   block, 0, s, t, T, lcp, g_mag, mag, ws, status, itmax, epl);       \
                 break;                                                                           \
             case 4:                                                                              \
                 hipLaunchKernelGGL((k_lightcurve_point_source<NL, MODE_CF32, false>), grid,      \
-                                   block, 0, s, t, T, lcp, g_mag, mag, ws, status, itmax);       \
+                                   block, 0, s, t, T, lcp, g_mag, mag, ws, status, itmax, epl);       \
                 break;                                                                           \
             case 5:                                                                              \
                 hipLaunchKernelGGL((k_lightcurve_point_source<NL, MODE_CF32, true>), grid,       \
-                                   block, 0, s, t, T, lcp, g_mag, mag, ws, status, itmax);       \
+                                   block, 0, s, t, T, lcp, g_mag, mag, ws, status, itmax, epl);       \
                 break;                                                                           \
             default:                                                                             \
                 return CAUS_ERR_ARG;                                                             \

@@ -107,16 +107,21 @@ __device__ __forceinline__ cd aberth_sum(const cd (&z)[DEG], int j, cd x) {
 }
 
 // ---- EFT building blocks (horner.h:37-84, 164-217) ---------------------------
+// Error-free transformations must round every operation separately: FMA contraction
+// across these statements (a product feeding a later sum) would break the error terms.
 __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+#pragma clang fp contract(off)
     s = a + b;
     double t = s - a;
     e = (a - (s - t)) + (b - t);
 }
 __device__ __forceinline__ void two_prod(double a, double b, double& p, double& e) {
+#pragma clang fp contract(off)
     p = a * b;
     e = fma(a, b, -p);
 }
 __device__ __forceinline__ void two_prod_c(cd a, cd b, cd& r, cd& e1, cd& e2, cd& e3) {
+#pragma clang fp contract(off)
     double p0, q0, p1, q1, p2, q2, p3, q3, s4, r4, s5, r5;
     two_prod(a.re, b.re, p0, q0);
     two_prod(a.im, b.im, p1, q1);
@@ -142,6 +147,7 @@ __device__ __forceinline__ void cswap_desc(double& a, double& b) {
 // Priest's doubly compensated sum of four terms sorted by decreasing magnitude
 // (horner.h:164-208); the sort is a 5-comparator network.
 __device__ __forceinline__ double priest4(double p0, double p1, double p2, double p3) {
+#pragma clang fp contract(off)
     cswap_desc(p0, p1);
     cswap_desc(p2, p3);
     cswap_desc(p0, p2);
@@ -170,6 +176,7 @@ __device__ __forceinline__ double gamma_const(int n) {
 template <int DEG, bool REV>
 __device__ __forceinline__ void horner_comp(const cd (&p)[DEG + 1], cd x, cd& h, cd& hd,
                                             double& eb) {
+#pragma clang fp contract(off)
     cd e = cd{0, 0}, ed = cd{0, 0};
     h = REV ? p[0] : p[DEG];
     hd = cd{0, 0};

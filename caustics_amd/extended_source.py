@@ -129,6 +129,7 @@ def extended_fwd_vjp(w0, rho, g_mag, nlenses=2, npts_limb=150, limb_darkening=Fa
     prm = torch.stack([_col(params[k], S) for k in names], -1).contiguous() if names else None
     npts_arr = np.asarray(npts_limb)
     npts_t = None if npts_arr.ndim == 0 else _col(npts_arr, S, torch.int32)
-    return _run(nlenses, w, _col(rho, S), prm, npts_t, int(npts_arr.max()), limb_darkening,
-                _col(u1, S) if limb_darkening else None, npts_ld, roots_itmax, _lib.MODES[mode],
-                g=_col(g_mag, S))
+    mag, status, g_out = _run(nlenses, w, _col(rho, S), prm, npts_t, int(npts_arr.max()),
+                              limb_darkening, _col(u1, S) if limb_darkening else None, npts_ld,
+                              roots_itmax, _lib.MODES[mode], g=_col(g_mag, S))
+    return mag, g_out, status

@@ -156,7 +156,7 @@ def lens_eq_det_jac(z, nlenses=2, **params):
 def _critical_coeffs(phi, nlenses, a, e1, e2=None, r3=None):
     """Highest-first coefficients of prod_j (z - r_j)^2 exp(-i phi) - sum_j e_j
     prod_{k!=j} (z - r_k)^2, the critical-curve polynomial (point_source.py:1482-1534)."""
-    x = torch.exp(-1j * phi)
+    x = torch.exp(-1j * phi) * torch.ones_like(a)   # (B, npts)
     one = torch.ones_like(x)
     if nlenses == 2:
         return torch.stack([x, 0 * x, -2 * a**2 * x - 1.0, (-4 * a * e1 + 2 * a) * one,

@@ -172,8 +172,13 @@ def lens_eq_det_jac(z, nlenses=2, reference_single_lens=False, **lp):
                         + (1.0 - e1 - e2) / (zb - np.conj(r3)) ** 2) ** 2
 
 
-def images_point_source(w, nlenses=2, itmax=2500, compensated=False, init_mode=0, **lp):
-    """(n, *w.shape) images and real-image mask (point_source.py:1652-1709)."""
+def images_point_source(w, nlenses=2, itmax=2500, compensated=False, init_mode=0, perturb=0.0,
+                        seed=0, **lp):
+    """(n, *w.shape) images and real-image mask (point_source.py:1652-1709).
+
+    perturb > 0 multiplies every coefficient by (1 + perturb*(u + i v)), u, v uniform in
+    [-1, 1] (seeded): the spread of results under perturb ~ 1e-16 measures how many
+    digits the FP64 polynomial route can determine at all."""
     w = np.asarray(w, complex)
     if nlenses == 1:
         wa = w.real**2 + w.imag**2
@@ -182,18 +187,22 @@ def images_point_source(w, nlenses=2, itmax=2500, compensated=False, init_mode=0
         z = np.stack([z1, z2])
         return z, np.ones(z.shape, bool)
     c = poly_coeffs(w, nlenses, **lp)
+    if perturb:
+        g = np.random.default_rng(seed)
+        c = c * (1 + perturb * (g.uniform(-1, 1, c.shape) + 1j * g.uniform(-1, 1, c.shape)))
     z = ea.poly_roots(c, itmax=itmax, compensated=compensated, init_mode=init_mode)
     z = np.moveaxis(z, -1, 0)
     mask = np.abs(lens_eq(z, nlenses, **lp) - w) < 1e-6
     return z, mask
 
 
-def mag_point_source(w, nlenses=2, itmax=2500, compensated=False, init_mode=0, **params):
+def mag_point_source(w, nlenses=2, itmax=2500, compensated=False, init_mode=0, perturb=0.0,
+                     seed=0, **params):
     """Point-source magnification (point_source.py:1762-1830)."""
     lp, x_cm = lens_params(nlenses, **params)
     w = np.asarray(w, complex) + x_cm
     z, mask = images_point_source(w, nlenses, itmax=itmax, compensated=compensated,
-                                  init_mode=init_mode, **lp)
+                                  init_mode=init_mode, perturb=perturb, seed=seed, **lp)
     det = lens_eq_det_jac(z, nlenses, **lp)
     return ((1.0 / np.abs(det)) * mask).sum(axis=0)
 

@@ -152,13 +152,13 @@ def planetary_caustic_test(w, rho, c_p=2.0, **lp):
     return (w_pc - w).real ** 2 + (w_pc - w).imag ** 2 > c_p * (rho**2 + d_pc**2)
 
 
-def hexadecapole_gate(w_points, rho, nlenses=2, **params):
+def hexadecapole_gate(w_points, rho, nlenses=2, perturb=0.0, seed=0, **params):
     """The per-point decision of mag() (lightcurve.py:168-227): returns
     (mu_hex, use_hexadecapole) with the reference's choices (no u1 in the multipole
     terms, planetary test for q < 0.01).  nlenses=3 always needs the contour."""
     lp, x_cm = lens.lens_params(nlenses, **params)
     wl = np.asarray(w_points, complex) + x_cm
-    z, mask = lens.images_point_source(wl, nlenses, **lp)
+    z, mask = lens.images_point_source(wl, nlenses, perturb=perturb, seed=seed, **lp)
     mu, dmu = mag_hexadecapole(z, mask, rho, 0.0, nlenses, **lp)
     if nlenses == 3:
         return mu, np.zeros(wl.shape, bool)

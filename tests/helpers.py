@@ -59,3 +59,12 @@ def agree_condition_aware(test, plain, truth, rel=1e-10, factor=10.0):
     e_t = np.abs(test - truth) / np.abs(truth)
     e_p = np.abs(plain - truth) / np.abs(truth)
     return (e_tp <= rel) | (e_t <= np.maximum(rel, factor * e_p))
+
+
+def agree_within_spread(test, plain, perturbed, rel=1e-10, factor=100.0):
+    """Per item: |test - plain| <= max(rel |plain|, factor * spread), where `perturbed`
+    holds oracle results for coefficients perturbed at the 1e-16 level (a list of
+    arrays): the spread is how far the FP64 polynomial route itself is determined."""
+    test, plain = np.asarray(test), np.asarray(plain)
+    spread = np.max([np.abs(np.asarray(p) - plain) for p in perturbed], axis=0)
+    return np.abs(test - plain) <= np.maximum(rel * np.abs(plain), factor * spread)

@@ -6,14 +6,7 @@ import pytest
 from caustics_amd.lightcurve import hexadecapole_gate
 from oracle import lens, multipole
 
-from helpers import agree_condition_aware
-
-
-def gate_truth(w, rho, **params):
-    """Hexadecapole magnification from compensated-arithmetic images."""
-    lp, x_cm = lens.lens_params(2, **params)
-    z, mask = lens.images_point_source(np.asarray(w) + x_cm, 2, compensated=True, **lp)
-    return multipole.mag_hexadecapole(z, mask, rho, 0.0, 2, **lp)[0]
+from helpers import agree_within_spread
 
 pytestmark = pytest.mark.gpu
 
@@ -31,8 +24,10 @@ def test_gate_matches_oracle_near_caustics(gpu, q):
     e = np.abs(mu - mo) / np.abs(mo)
     print(f"q={q}: accepted {use.mean():.2f}; mu max rel err {e.max():.2e}; "
           f"decision agreement {(use == uo).mean():.4f}")
-    ok = agree_condition_aware(mu, mo, gate_truth(w, rho, s=s, q=q))
-    print(f"condition-aware agreement {ok.mean():.4f}")
+    pert = [multipole.hexadecapole_gate(w, rho, nlenses=2, perturb=4e-16, seed=sd, s=s, q=q)[0]
+            for sd in (1, 2)]
+    ok = agree_within_spread(mu, mo, pert)
+    print(f"agreement within the oracle's own coefficient spread {ok.mean():.4f}")
     assert ok.mean() >= 0.999
     assert (use == uo).mean() >= 0.995
 

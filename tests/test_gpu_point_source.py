@@ -15,7 +15,7 @@ from caustics_amd.lightcurve import lightcurve_fwd_vjp, mag_point_source_lightcu
 from caustics_amd.point_source import mag_point_source
 from oracle import lens
 
-from helpers import agree_condition_aware
+from helpers import agree_within_spread
 
 pytestmark = pytest.mark.gpu
 
@@ -67,15 +67,17 @@ def test_triple_matches_oracle(gpu):
     p = W.lens_configs(g, 3000, 3)
     w = W.disk(g, 3000)
     m = mag_point_source(w, nlenses=3, **p).cpu().numpy()
-    def orc(comp):
+    def orc(perturb, seed=0):
         return np.array([lens.mag_point_source(w[i], nlenses=3, s=p["s"][i], q=p["q"][i],
                                                q3=p["q3"][i], r3=p["r3"][i], psi=p["psi"][i],
-                                               compensated=comp).item() for i in range(3000)])
-    mo, mt = orc(False), orc(True)
+                                               perturb=perturb, seed=seed + i).item()
+                         for i in range(3000)])
+    mo = orc(0.0)
+    pert = [orc(4e-16, 1), orc(4e-16, 7)]
     e = rel(m, mo)
-    ok = agree_condition_aware(m, mo, mt)
+    ok = agree_within_spread(m, mo, pert)
     print(f"triple: frac within 1e-10 of the FP64 oracle {np.mean(e <= 1e-10):.4f}; "
-          f"condition-aware agreement {ok.mean():.4f}")
+          f"within the oracle's own 1-ulp coefficient spread {ok.mean():.4f}")
     assert ok.mean() >= 0.999
 
 
@@ -109,17 +111,20 @@ def test_gradients_vs_oracle_and_fd(gpu, nlenses):
     # central differences on s and Re(w) (h = 1e-6)
     h = 1e-6
     for k, key in ((2, "s"), (0, "w")):
-        if key == "w":
-            mp = lens.mag_point_source(w + h, nlenses=nlenses, **p)
-            mm = lens.mag_point_source(w - h, nlenses=nlenses, **p)
-        else:
-            pp, pm = dict(p), dict(p)
-            pp[key] = p[key] + h
-            pm[key] = p[key] - h
-            mp = lens.mag_point_source(w, nlenses=nlenses, **pp)
-            mm = lens.mag_point_source(w, nlenses=nlenses, **pm)
-        fd = (mp - mm) / (2 * h)
-        e = np.abs(fd - ours[:, k]) / np.maximum(np.abs(fd), 1e-2 * mo)
+        fd = np.zeros(n)
+        for i in range(n):
+            pi = {kk: p[kk][i] for kk in names}
+            if key == "w":
+                mp = lens.mag_point_source(w[i] + h, nlenses=nlenses, **pi)
+                mm = lens.mag_point_source(w[i] - h, nlenses=nlenses, **pi)
+            else:
+                pp, pm = dict(pi), dict(pi)
+                pp[key] += h
+                pm[key] -= h
+                mp = lens.mag_point_source(w[i], nlenses=nlenses, **pp)
+                mm = lens.mag_point_source(w[i], nlenses=nlenses, **pm)
+            fd[i] = (mp - mm).item() / (2 * h)
+        e = (np.abs(fd - ours[:, k]) / np.maximum(np.abs(fd), 1e-2 * mo))[well]
         assert np.quantile(e, 0.99) < 1e-4
 
 
@@ -135,11 +140,12 @@ def test_lightcurve_fwd_and_vjp(gpu):
     wtraj = W.trajectory(t, prm["t0"], prm["u0"], prm["tE"], prm["alpha"])
     mo = np.stack([lens.mag_point_source(wtraj[l], nlenses=2, s=prm["s"][l], q=prm["q"][l])
                    for l in range(L)])
-    mt = np.stack([lens.mag_point_source(wtraj[l], nlenses=2, s=prm["s"][l], q=prm["q"][l],
-                                         compensated=True) for l in range(L)])
-    ok = agree_condition_aware(mag, mo, mt)
+    pert = [np.stack([lens.mag_point_source(wtraj[l], nlenses=2, s=prm["s"][l], q=prm["q"][l],
+                                            perturb=4e-16, seed=sd + l) for l in range(L)])
+            for sd in (1, 1000)]
+    ok = agree_within_spread(mag, mo, pert)
     print(f"light curves: within 1e-10 {np.mean(rel(mag, mo) <= 1e-10):.5f}, "
-          f"condition-aware {ok.mean():.5f}")
+          f"within the oracle's coefficient spread {ok.mean():.5f}")
     assert ok.mean() > 0.9999
     # autograd path gives the same VJP
     cols = {k: torch.as_tensor(prm[k], device="cuda").requires_grad_(True)
