@@ -60,6 +60,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--roots-only", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (a kernel of "
+                         "known traffic, to scale hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (every unit of the path is "
@@ -88,6 +91,14 @@ def main():
         rc = L.caus_roots(_lib.stream_ptr(), _lib.ptr(c), None, _lib.ptr(roots[nl]), None,
                           _lib.ptr(status[nl]), c.shape[0], degs[nl], 2000, mode, init)
         _lib.check(rc, "caus_roots")
+
+    if args.copy_mib:
+        src = torch.ones(args.copy_mib << 17, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"bench: copied {args.copy_mib} MiB", file=sys.stderr, flush=True)
+        del src, dst
 
     forms = {"fp64": (_lib.MODE_FP64, _lib.INIT_COMPLEX),
              "cf32": (_lib.MODE_CF32, _lib.INIT_COMPLEX),
@@ -209,18 +220,26 @@ def main():
     print(json.dumps(line, default=float))
 
 
-def _timed(fn, steps, warmup):
-    """Run fn warmup times, then time `steps` calls between device synchronisations."""
+def _timed(fn, steps, warmup, name="", budget_s=60.0):
+    """Run fn warmup times, then time `steps` calls between device synchronisations
+    (fewer if one call is so slow that `steps` would exceed budget_s)."""
     import torch
 
-    for _ in range(warmup):
+    for i in range(max(1, warmup)):
+        t0 = time.perf_counter()
         fn()
-    torch.cuda.synchronize()
+        torch.cuda.synchronize()
+        dt1 = time.perf_counter() - t0
+        print(f"bench: {name} warmup {i + 1}: {dt1:.3f} s", file=sys.stderr, flush=True)
+    steps = max(1, min(steps, int(budget_s / max(dt1, 1e-9))))
     t0 = time.perf_counter()
     for _ in range(steps):
         fn()
     torch.cuda.synchronize()
-    return (time.perf_counter() - t0) / steps
+    dt = (time.perf_counter() - t0) / steps
+    print(f"bench: {name} {steps} timed steps, {dt * 1e3:.3f} ms/step", file=sys.stderr,
+          flush=True)
+    return dt
 
 
 def bench_point_source(steps, warmup):
@@ -240,7 +259,7 @@ def bench_point_source(steps, warmup):
     def run():
         _lib.check(L.caus_mag_point_source(_lib.stream_ptr(), 2, _lib.ptr(w), n, _lib.ptr(prm), 0,
                                            _lib.ptr(mag), None, 2500, _lib.MODE_FP64), "ps")
-    dt = _timed(run, steps * 10, warmup)
+    dt = _timed(run, steps * 10, warmup, "C1 point source")
     return {"workload": "10^4 binary point-source magnifications (s=0.9, q=0.1), forward, FP64",
             "mag_evals_per_s": n / dt, "ms": dt * 1e3}
 
@@ -272,7 +291,7 @@ def bench_lightcurves(steps, warmup, host_check=True):
             _lib.check(Lb.caus_lightcurve_point_source(
                 _lib.stream_ptr(), 2, _lib.ptr(t), T, _lib.ptr(lcp), L_, _lib.ptr(g),
                 _lib.ptr(mag), _lib.ptr(gp), _lib.ptr(ws), wsb, None, 2500, mode), "lc")
-        dt = _timed(run, steps, warmup)
+        dt = _timed(run, steps, warmup, f"C3 light curves {name}")
         run()
         torch.cuda.synchronize()
         mags[name] = (mag.clone(), gp.clone())
@@ -289,14 +308,14 @@ def bench_lightcurves(steps, warmup, host_check=True):
 def bench_extended_binary(steps, warmup, n_near=2048, n_far=2048):
     """Binary limb-darkened (u=0.5) sources, rho log-uniform [1e-3,1e-1]: n_near within
     2 rho of a caustic with ragged contour resolution 256..4096 points, n_far accepted by
-    the hexadecapole gate.  One step: gate on all sources, then the contour kernel with
-    its VJP on every source the gate rejects."""
+    the hexadecapole gate.  One step is mag() forward + VJP: the gate on all sources,
+    the hexadecapole VJP on the accepted ones and the contour kernel with its VJP on
+    every source the gate rejects."""
     import torch
 
     from caustics_amd import _lib
     from caustics_amd import workloads as W
-    from caustics_amd.extended_source import _run
-    from caustics_amd.lightcurve import hexadecapole_gate
+    from caustics_amd.lightcurve import _mag_fwd
 
     b = W.extended_binary_batch(n_near, n_far)
     S = len(b["w"])
@@ -307,24 +326,20 @@ def bench_extended_binary(steps, warmup, n_near=2048, n_far=2048):
     npts_all = np.concatenate([b["npts"], np.full(n_far, 256)])
     npts = torch.as_tensor(npts_all, device="cuda", dtype=torch.int32)
     gm = torch.ones(S, dtype=torch.float64, device="cuda")
+    cfg = (2, npts, int(npts_all.max()), True, 100, 2500, _lib.MODE_FP64)
     res = {}
 
     def step():
-        mu, _, use = hexadecapole_gate(w, rho, nlenses=2, s=prm[:, 0], q=prm[:, 1])
-        idx = torch.nonzero(~use).reshape(-1)
-        m, st, go = _run(2, w[idx].contiguous(), rho[idx].contiguous(), prm[idx].contiguous(),
-                         npts[idx].contiguous(), int(npts_all.max()), True,
-                         u1[idx].contiguous(), 100, 2500, _lib.MODE_FP64, g=gm[idx].contiguous(),
-                         max_chunk=4096)
-        res["n_contour"] = int(idx.numel())
-        res["unconverged"] = int((st == 0).sum().item())
-    dt = _timed(step, steps, warmup)
+        m, use, go = _mag_fwd(cfg, w, rho, u1, prm, g=gm)
+        res["use"] = use
+        res["go"] = go
+    dt = _timed(step, steps, warmup, "C4 mag() fwd+VJP")
+    n_contour = int((~res["use"]).sum().item())
     return {"workload": f"{S} binary limb-darkened sources (u=0.5), {n_near} near caustics with "
-                        "ragged 256-4096 limb points, rest hexadecapole-gated; contour fwd+VJP",
-            "mag_evals_per_s": S / dt, "contour_sources": res["n_contour"],
-            "contour_fwd_vjp_evals_per_s": res["n_contour"] / dt, "ms": dt * 1e3,
-            "unconverged": res["unconverged"],
-            "note": "hexadecapole-gated sources are forward only (no VJP of the hexadecapole)"}
+                        "ragged 256-4096 limb points, rest hexadecapole-gated; mag() fwd+VJP",
+            "mag_evals_per_s": S / dt, "contour_sources": n_contour,
+            "hexadecapole_sources": S - n_contour, "ms": dt * 1e3,
+            "grad_finite": bool(torch.isfinite(res["go"]).all().item())}
 
 
 def bench_triple(steps, warmup, n_sets=8, n_epochs=1500, npts=256):
@@ -349,7 +364,7 @@ def bench_triple(steps, warmup, n_sets=8, n_epochs=1500, npts=256):
         m, st, go = _run(3, w, rho, prm, None, npts, True, u1, 100, 2500, _lib.MODE_FP64, g=gm,
                          max_chunk=S)
         st_holder["st"] = st
-    dt = _timed(step, steps, warmup)
+    dt = _timed(step, steps, warmup, "C5 triple contours")
     return {"workload": f"{n_sets} triple-lens parameter sets x {n_epochs} epochs (caustic "
                         f"crossing), limb-darkened u=0.4, npts_limb={npts}, full contour fwd+VJP",
             "mag_evals_per_s_fwd_vjp": S / dt, "ms": dt * 1e3,
@@ -380,9 +395,22 @@ def cpu_baseline(host, seconds):
         total_t += time.perf_counter() - t0
         total_n += c5.shape[0] + c10.shape[0]
         off += chunk5
-    return {"value": total_n / total_t, "unit": "polys/s", "cores": 1, "kind": kind,
-            "sample": f"{total_n} polynomials ({off} degree-5 + {off // 4} degree-10) of the "
-                      f"same workload, FP64, plain Ehrlich-Aberth, {total_t:.1f} s"}
+    out = {"value": total_n / total_t, "unit": "polys/s", "cores": 1, "kind": kind,
+           "sample": f"{total_n} polynomials ({off} degree-5 + {off // 4} degree-10) of the "
+                     f"same workload, FP64, plain Ehrlich-Aberth, {total_t:.1f} s"}
+    # the C restatement with OpenMP over polynomials on every core this process may use
+    ncores = min(len(os.sched_getaffinity(0)), 16)
+    n5 = min(host[2].shape[0], max(chunk5, int(out["value"] * seconds * ncores * 0.4) // 5 * 4))
+    c5, c10 = host[2][:n5], host[3][:n5 // 4]
+    ea.roots_asc(c5[:1024], nthreads=ncores)
+    t0 = time.perf_counter()
+    ea.roots_asc(c5, nthreads=ncores)
+    ea.roots_asc(c10, nthreads=ncores)
+    dt = time.perf_counter() - t0
+    out["all_cores"] = {"value": (n5 + n5 // 4) / dt, "unit": "polys/s", "cores": ncores,
+                        "kind": "port", "sample": f"{n5} degree-5 + {n5 // 4} degree-10, "
+                                                  f"{dt:.1f} s"}
+    return out
 
 
 if __name__ == "__main__":

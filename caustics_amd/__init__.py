@@ -6,7 +6,8 @@ All compute runs in hand-written HIP kernels for gfx950 (libcaustics_hip.so).
 """
 from .ehrlich_aberth_primitive import ehrlich_aberth, poly_roots
 from .extended_source import mag_extended_source
-from .lightcurve import hexadecapole_gate, mag, mag_point_source_lightcurve
+from .lightcurve import (hexadecapole_gate, hexadecapole_vjp, mag, mag_fwd_vjp,
+                         mag_point_source_lightcurve)
 from .point_source import critical_and_caustic_curves, lens_eq, lens_eq_det_jac, mag_point_source
 
 __all__ = [
@@ -19,5 +20,7 @@ __all__ = [
     "mag_point_source_lightcurve",
     "mag_extended_source",
     "hexadecapole_gate",
+    "hexadecapole_vjp",
     "mag",
+    "mag_fwd_vjp",
 ]

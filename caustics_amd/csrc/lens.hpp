@@ -223,6 +223,20 @@ __device__ __forceinline__ ImageTerms<NL> image_terms(cd z, const LensFrame<NL>&
 //   => dz = (u - K conj(u)) / det,   dK = -2 K3 conj(dz) + dK_theta,
 //   d det = -2 Re(conj(K) dK),       d(1/|det|) = -sign(det) d det / det^2.
 template <int NL>
+__device__ __forceinline__ cd image_dz(const ImageTerms<NL>& it, const LensFrame<NL>& lf,
+                                       double det, cd dw, const LensTangent<NL>& lt) {
+    cd dG = cd{0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < NL; ++j) {
+        const cd v = it.inv[j];
+        dG = dG + lt.de[j] * v;
+        dG = cfma(lf.e[j] * (v * v), conj(lt.dr[j]), dG);
+    }
+    const cd u = dw + dG;
+    return rcp(det) * (u - it.K * conj(u));
+}
+
+template <int NL>
 __device__ __forceinline__ double dmu_image(const ImageTerms<NL>& it, const LensFrame<NL>& lf,
                                             double det, cd dw, const LensTangent<NL>& lt) {
     cd dG = cd{0.0, 0.0}, dK = cd{0.0, 0.0};

@@ -17,7 +17,8 @@ import torch
 from . import _lib
 from .point_source import PARAM_NAMES, _mode_id
 
-__all__ = ["mag_point_source_lightcurve", "lightcurve_fwd_vjp", "TRAJ_NAMES"]
+__all__ = ["mag_point_source_lightcurve", "lightcurve_fwd_vjp", "hexadecapole_gate",
+           "hexadecapole_vjp", "mag", "mag_fwd_vjp", "TRAJ_NAMES"]
 
 TRAJ_NAMES = ("t0", "u0", "tE", "alpha")
 
@@ -110,26 +111,138 @@ def hexadecapole_gate(w_points, rho, nlenses=2, u1=0.0, roots_itmax=2500, **para
     return mu.reshape(shape), dmu.reshape(shape), use.reshape(shape).bool()
 
 
+def hexadecapole_vjp(w_points, rho, g_mu, nlenses=2, u1=0.0, roots_itmax=2500, **params):
+    """Hexadecapole magnification and g_mu * its derivative (HIP kernel): returns
+    (mu_hex (P,), g_out (P, 4 + n_params)) with g_out columns Re w, Im w, rho, u1 and the
+    lens parameters (reverse mode of src/caustics/multipole.py:208-244)."""
+    from .extended_source import _col
+
+    w = w_points if isinstance(w_points, torch.Tensor) else torch.as_tensor(np.asarray(w_points))
+    w = w.to(device="cuda", dtype=torch.complex128).reshape(-1).contiguous()
+    P = w.numel()
+    names = PARAM_NAMES[nlenses]
+    prm = torch.stack([_col(params[k], P) for k in names], -1).contiguous() if names else None
+    return _hex_vjp(nlenses, w, _col(rho, P), float(u1), prm, _col(g_mu, P), roots_itmax)
+
+
+def _hex_vjp(nlenses, w, rho_t, u1, prm, g, itmax):
+    P = w.numel()
+    NP = 0 if prm is None else prm.shape[1]
+    mu = torch.empty(P, dtype=torch.float64, device="cuda")
+    go = torch.empty((P, 4 + NP), dtype=torch.float64, device="cuda")
+    if P == 0:
+        return mu, go
+    rc = _lib.lib().caus_hexadecapole_vjp(_lib.stream_ptr(), nlenses, _lib.ptr(w), P,
+                                          _lib.ptr(prm), NP, _lib.ptr(rho_t), 1, float(u1),
+                                          _lib.ptr(g), _lib.ptr(mu), _lib.ptr(go), None,
+                                          int(itmax))
+    _lib.check(rc, "caus_hexadecapole_vjp")
+    return mu, go
+
+
+def _gate(nlenses, w, rho_t, prm, itmax):
+    P = w.numel()
+    mu = torch.empty(P, dtype=torch.float64, device="cuda")
+    use = torch.empty(P, dtype=torch.uint8, device="cuda")
+    NP = 0 if prm is None else prm.shape[1]
+    rc = _lib.lib().caus_hexadecapole_gate(_lib.stream_ptr(), nlenses, _lib.ptr(w), P,
+                                           _lib.ptr(prm), NP, _lib.ptr(rho_t), 1, 0.0,
+                                           _lib.ptr(mu), None, _lib.ptr(use), None, int(itmax))
+    _lib.check(rc, "caus_hexadecapole_gate")
+    return mu, use.bool()
+
+
+def _mag_fwd(cfg, w, rho_t, u1_t, prm, g=None):
+    """Gate, then the contour kernel on the rejected points; with g also the VJP of both
+    branches.  Returns (mag, use, g_out or None)."""
+    from .extended_source import _run
+
+    nlenses, npts_t, npts_max, ld, npts_ld, itmax, mode_id = cfg
+    P = w.numel()
+    NP = 0 if prm is None else prm.shape[1]
+    mu, use = _gate(nlenses, w, rho_t, prm, itmax)
+    out = mu.clone()
+    go = torch.zeros((P, 4 + NP), dtype=torch.float64, device="cuda") if g is not None else None
+    rej = torch.nonzero(~use).reshape(-1)
+    if g is not None:
+        acc = torch.nonzero(use).reshape(-1)
+        if acc.numel():
+            _, ga = _hex_vjp(nlenses, w[acc].contiguous(), rho_t[acc].contiguous(), 0.0,
+                             None if prm is None else prm[acc].contiguous(),
+                             g[acc].contiguous(), itmax)
+            ga[:, 3] = 0.0   # the gate's multipoles use no limb darkening
+            go[acc] = ga
+    if rej.numel():
+        sub = lambda x: None if x is None else x[rej].contiguous()
+        m, _, gr = _run(nlenses, sub(w), sub(rho_t), sub(prm), sub(npts_t), npts_max, ld,
+                        sub(u1_t) if ld else None, npts_ld, itmax, mode_id, g=sub(g))
+        out[rej] = m
+        if g is not None:
+            go[rej] = gr
+    return out, use, go
+
+
+class _Mag(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, cfg, w, rho_t, u1_t, *ps):
+        prm = torch.stack(list(ps), -1).contiguous() if ps else None
+        out, _, _ = _mag_fwd(cfg, w, rho_t, u1_t, prm)
+        ctx.cfg = cfg
+        ctx.save_for_backward(w, rho_t, u1_t, *ps)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        w, rho_t, u1_t, *ps = ctx.saved_tensors
+        prm = torch.stack(list(ps), -1).contiguous() if ps else None
+        _, _, go = _mag_fwd(ctx.cfg, w, rho_t, u1_t, prm, g.contiguous().to(torch.float64))
+        grads = [torch.complex(go[:, 0], go[:, 1]), go[:, 2], go[:, 3]]
+        grads += [go[:, 4 + k] for k in range(len(ps))]
+        return (None, *grads)
+
+
+def _mag_args(w_points, rho, nlenses, npts_limb, limb_darkening, u1, npts_ld, roots_itmax,
+              roots_compensated, mode, params):
+    from .extended_source import _col
+
+    if nlenses not in (1, 2, 3):
+        raise ValueError("`nlenses` has to be set to be <= 3.")
+    if roots_compensated:
+        raise NotImplementedError("the contour kernels run the FP64 or CF32 solver")
+    w = w_points if isinstance(w_points, torch.Tensor) else torch.as_tensor(np.asarray(w_points))
+    shape = tuple(w.shape)
+    w = w.to(device="cuda", dtype=torch.complex128).reshape(-1).contiguous()
+    P = w.numel()
+    names = PARAM_NAMES[nlenses]
+    npts_arr = np.asarray(npts_limb)
+    npts_t = None if npts_arr.ndim == 0 else _col(npts_arr, P, torch.int32)
+    cfg = (nlenses, npts_t, int(npts_arr.max()), bool(limb_darkening), int(npts_ld),
+           int(roots_itmax), _lib.MODES[mode or "fp64"])
+    return cfg, shape, w, _col(rho, P), _col(u1, P), [_col(params[k], P) for k in names]
+
+
 def mag(w_points, rho, nlenses=2, npts_limb=200, limb_darkening=False, u1=0.0, npts_ld=100,
         roots_itmax=2500, roots_compensated=False, mode=None, **params):
     """Extended-source magnification at source positions w_points (reference:
     src/caustics/lightcurve.py:88-254): the hexadecapole value where the gate accepts
     it, the full contour integral elsewhere.  As in the reference the multipole terms
-    of the gate use no limb darkening; three-lens systems always use the contour."""
-    from .extended_source import _col, mag_extended_source
+    of the gate use no limb darkening; three-lens systems always use the contour.
+    Differentiable in w_points, rho, u1 and the lens parameters (hexadecapole VJP on
+    the accepted points, contour VJP on the others)."""
+    cfg, shape, w, rho_t, u1_t, ps = _mag_args(w_points, rho, nlenses, npts_limb,
+                                               limb_darkening, u1, npts_ld, roots_itmax,
+                                               roots_compensated, mode, params)
+    return _Mag.apply(cfg, w, rho_t, u1_t, *ps).reshape(shape)
 
-    w = w_points if isinstance(w_points, torch.Tensor) else torch.as_tensor(np.asarray(w_points))
-    shape = tuple(w.shape)
-    w = w.to(device="cuda", dtype=torch.complex128).reshape(-1).contiguous()
-    P = w.numel()
-    mu, _, use = hexadecapole_gate(w, rho, nlenses, 0.0, roots_itmax, **params)
-    out = mu.clone()
-    idx = torch.nonzero(~use).reshape(-1)
-    if idx.numel():
-        names = PARAM_NAMES[nlenses]
-        sub = {k: _col(params[k], P)[idx] for k in names}
-        out[idx] = mag_extended_source(w[idx], _col(rho, P)[idx], nlenses=nlenses,
-                                       npts_limb=npts_limb, limb_darkening=limb_darkening,
-                                       u1=_col(u1, P)[idx], npts_ld=npts_ld,
-                                       roots_itmax=roots_itmax, mode=mode, **sub)
-    return out.reshape(shape)
+
+def mag_fwd_vjp(w_points, rho, g_mag, nlenses=2, npts_limb=200, limb_darkening=False, u1=0.0,
+                npts_ld=100, roots_itmax=2500, mode=None, **params):
+    """Fused forward + VJP of mag(): (mag (P,), g_out (P, 4+n_params), use_hex (P,)), g_out
+    columns Re w, Im w, rho, u1 and the lens parameters."""
+    cfg, shape, w, rho_t, u1_t, ps = _mag_args(w_points, rho, nlenses, npts_limb,
+                                               limb_darkening, u1, npts_ld, roots_itmax,
+                                               False, mode, params)
+    from .extended_source import _col
+
+    prm = torch.stack(ps, -1).contiguous() if ps else None
+    return _mag_fwd(cfg, w, rho_t, u1_t, prm, _col(g_mag, w.numel()))

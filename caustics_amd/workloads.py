@@ -159,13 +159,24 @@ def extended_binary_batch(n_near, n_far, seed=4, npts_choices=(256, 512, 1024, 2
     w_near = pick + r * np.exp(1j * g.uniform(-np.pi, np.pi, n_near))
     w_far = np.zeros(n_far, complex)
     todo = np.arange(n_far)
-    while len(todo):
+    for attempt in range(400):
+        if not len(todo):
+            break
+        if attempt and attempt % 8 == 0:
+            # some (s, q, rho) admit no gate-accepted point in |w| <= 2 (the quadrupole
+            # term alone exceeds the gate's 1e-2 for large rho): redraw those systems
+            k = n_near + todo
+            pr = lens_configs(g, len(todo), 2)
+            s[k], q[k] = pr["s"], pr["q"]
+            rho[k] = loguniform(g, 1e-3, 1e-1, len(todo))
         cand = disk(g, len(todo))
         _, _, use = hexadecapole_gate(cand, rho[n_near + todo], nlenses=2, s=s[n_near + todo],
                                       q=q[n_near + todo])
         ok = use.cpu().numpy()
         w_far[todo[ok]] = cand[ok]
         todo = todo[~ok]
+    if len(todo):
+        raise RuntimeError(f"{len(todo)} far sources found no gate-accepted position")
     npts = g.choice(np.asarray(npts_choices), n_near)
     return dict(w=np.concatenate([w_near, w_far]), rho=rho, s=s, q=q, u1=np.full(len(rho), u1),
                 npts=npts, n_near=n_near)

@@ -132,6 +132,19 @@ int caus_hexadecapole_gate(void* stream, int32_t nlenses, const double* w, int64
                            int64_t rho_stride, double u1, double* mu_hex, double* delta_mu,
                            uint8_t* use_hex, uint8_t* status, int32_t itmax);
 
+/* Reverse mode of the hexadecapole magnification (the derivative the reference gets by
+ * JAX autodiff of _mag_hexadecapole, src/caustics/multipole.py:208-244, on the points
+ * mag() accepts, src/caustics/lightcurve.py:215-254).  Arguments as for
+ * caus_hexadecapole_gate, plus
+ *   g_mu   (P,) cotangent of mu_hex
+ *   mu_hex (P,) out: the hexadecapole magnification (same value as the gate's)
+ *   g_out  (P, 4 + n_params) out: g_mu * d mu_hex / d(Re w, Im w, rho, u1, lens params),
+ *          n_params = 0, 2, 5 for nlenses = 1, 2, 3 */
+int caus_hexadecapole_vjp(void* stream, int32_t nlenses, const double* w, int64_t P,
+                          const double* params, int64_t params_stride, const double* rho,
+                          int64_t rho_stride, double u1, const double* g_mu, double* mu_hex,
+                          double* g_out, uint8_t* status, int32_t itmax);
+
 /* Extended-source magnification by contour integration, one workgroup per source
  * (reference: mag_extended_source, src/caustics/extended_source.py:730-901, with
  * _images_of_source_limb :56-153, segments and stitching :156-727 and the integrals of

@@ -108,7 +108,14 @@ def limb_thetas_initial(npts):
     return np.append(th, np.pi - 1e-8)
 
 
-def images_of_source_limb(w0, rho, nlenses, npts, lp, niter=LIMB_NITER):
+def images_of_source_limb(w0, rho, nlenses, npts, lp, niter=LIMB_NITER, theta=None):
+    """theta given: images at exactly these limb angles (sorted, no refinement) with the
+    same identity continuation -- the discretisation of an earlier call, frozen, so
+    that finite differences see only the smooth dependence on the inputs."""
+    if theta is not None:
+        z, mask, par = _solve_points(w0, rho, np.asarray(theta), nlenses, lp)
+        z, mask, par = chain_permute(z, mask, par)
+        return z, mask, par, np.asarray(theta)
     theta = limb_thetas_initial(npts)
     z, mask, par = _solve_points(w0, rho, theta, nlenses, lp)
     z, mask, par = chain_permute(z, mask, par)
@@ -352,10 +359,10 @@ def integrate_ld(z, tidx, w0, rho, u1, nlenses, lp, npts=100):
 # magnification
 # ---------------------------------------------------------------------------
 def mag_extended_source(w0, rho, nlenses=2, npts_limb=150, limb_darkening=False, u1=0.0,
-                        npts_ld=100, return_parts=False, **params):
+                        npts_ld=100, return_parts=False, theta=None, **params):
     lp, x_cm = lens.lens_params(nlenses, **params)
     w0 = complex(w0) + x_cm
-    z, mask, par, theta = images_of_source_limb(w0, rho, nlenses, npts_limb, lp)
+    z, mask, par, theta = images_of_source_limb(w0, rho, nlenses, npts_limb, lp, theta=theta)
     if limb_darkening:
         integ = lambda c, t: integrate_ld(c, t, w0, rho, u1, nlenses, lp, npts_ld)
     else:

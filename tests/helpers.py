@@ -39,14 +39,47 @@ def residual_ratio(coeffs_asc, z):
     return np.abs(pz) / den
 
 
-def roots_close(z_test, z_ref, coeffs_asc, rel=1e-12, kappa_factor=64.0):
+def roots_close(z_test, z_ref, coeffs_asc, rel=1e-12, kappa_factor=64.0, truth=True):
     """Matched root sets agree to `rel` relative, or within kappa_factor*kappa*eps
-    for ill-conditioned roots.  Returns (ok mask, matched z_test, rel error)."""
+    for ill-conditioned roots.  With truth=True a row that fails is judged against
+    its 40-digit roots instead: the tested roots must be within `rel`, the kappa
+    bound, or 10x the oracle's own error of them (first-order kappa underestimates
+    the error of nearly multiple roots).  Returns (ok mask, matched z_test, rel error)."""
     zt = match_rows(z_ref, z_test)
     err = np.abs(zt - z_ref) / np.maximum(np.abs(z_ref), 1e-300)
     kap = root_condition(coeffs_asc, z_ref)
     tol = np.maximum(rel, kappa_factor * kap * EPS)
-    return err <= tol, zt, err
+    ok = err <= tol
+    bad = np.nonzero(~ok.all(axis=1))[0]
+    if truth and bad.size:
+        from oracle import mp_lens
+
+        zt_true = match_rows(z_ref[bad], mp_lens.roots_mp(np.asarray(coeffs_asc)[bad]))
+        a = np.maximum(np.abs(zt_true), 1e-300)
+        e_test = np.abs(match_rows(zt_true, zt[bad]) - zt_true) / a
+        e_ref = np.abs(z_ref[bad] - zt_true) / a
+        ok[bad] = e_test <= np.maximum(tol[bad], 10.0 * e_ref)
+    return ok, zt, err
+
+
+def agree_with_truth(test, oracle_runs, truth_fn, rel=1e-10, factor=10.0):
+    """Per item: |test - oracle| <= rel |oracle|; where that fails the item is judged
+    against the 40-digit value truth_fn(indices) -> (truth, ambiguous): it passes if
+    ambiguous (an image within a factor 10 of the 1e-6 filter) or if
+    |test - truth| <= max(rel |truth|, factor * max_k |oracle_k - truth|), the oracle
+    runs being the plain FP64 oracle and copies with 1-ulp coefficient perturbations
+    (the size of FP64 error the route itself has at that point)."""
+    test = np.asarray(test, float)
+    runs = [np.asarray(r, float) for r in oracle_runs]
+    ok = np.abs(test - runs[0]) <= rel * np.abs(runs[0])
+    idx = np.nonzero(~ok.reshape(-1))[0]
+    if idx.size:
+        tr, amb = truth_fn(idx)
+        tf, rf = test.reshape(-1)[idx], [r.reshape(-1)[idx] for r in runs]
+        scale = np.max([np.abs(r - tr) for r in rf], axis=0)
+        good = amb | (np.abs(tf - tr) <= np.maximum(rel * np.abs(tr), factor * scale))
+        ok.reshape(-1)[idx] = good
+    return ok
 
 
 def agree_condition_aware(test, plain, truth, rel=1e-10, factor=10.0):

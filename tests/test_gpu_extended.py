@@ -109,8 +109,12 @@ def _fd(fun, x0, h):
 
 @pytest.mark.parametrize("ld", [False, True])
 def test_extended_vjp_vs_finite_differences(gpu, ld):
-    # the reference's gradient test (tests/test_extended_source.py:294-331): d mag / d
-    # (s, q, rho, u1) at a caustic point, rtol 1e-3 against finite differences
+    """d mag / d(Re w0, Im w0, rho, u1, s, q) near caustics against central differences
+    of the forward kernel.  The discretised contour integral is only piecewise smooth
+    (image counts and segment ends change as a limb point crosses the caustic), so the
+    step is small (h = 1e-9, as the reference's test) and a point counts only where the
+    differences at h and 2h agree to 1e-4; there the tolerance is the reference's rtol
+    1e-3 (tests/test_extended_source.py:294-331)."""
     from caustics_amd.extended_source import extended_fwd_vjp
 
     s, q, rho, u1 = 0.9, 0.2, 1e-2, 0.7
@@ -118,29 +122,61 @@ def test_extended_vjp_vs_finite_differences(gpu, ld):
     kw = dict(nlenses=2, npts_limb=300, limb_darkening=ld, npts_ld=100)
     mag, g, st = extended_fwd_vjp(w, rho, np.ones(len(w)), u1=u1, s=s, q=q, **kw)
     g = g.cpu().numpy()
-    f = lambda **o: mag_extended_source(w, o.get("rho", rho), u1=o.get("u1", u1),
-                                        s=o.get("s", s), q=o.get("q", q), **kw).cpu().numpy()
-    h = 1e-7
-    checks = {"rho": 2, "s": 4, "q": 5}
+    mag = mag.cpu().numpy()
+    base = dict(w=w, rho=rho, s=s, q=q, u1=u1)
+
+    def f(name, h):
+        o = dict(base)
+        if name == "re":
+            o["w"] = w + h
+        elif name == "im":
+            o["w"] = w + 1j * h
+        else:
+            o[name] = base[name] + h
+        return mag_extended_source(o["w"], o["rho"], u1=o["u1"], s=o["s"], q=o["q"],
+                                   **kw).cpu().numpy()
+
+    checks = {"re": 0, "im": 1, "rho": 2, "s": 4, "q": 5}
     if ld:
         checks["u1"] = 3
     for name, col in checks.items():
-        base = dict(rho=rho, s=s, q=q, u1=u1)
-        bp, bm = dict(base), dict(base)
-        bp[name] += h
-        bm[name] -= h
-        fd = (f(**bp) - f(**bm)) / (2 * h)
-        err = np.abs(g[:, col] - fd) / np.maximum(np.abs(fd), 1e-3 * mag.cpu().numpy())
-        print(name, "max rel err vs FD", err.max())
-        assert np.median(err) < 1e-3
-        assert np.mean(err < 1e-2) >= 0.8
-    # source position
-    fw = lambda d: mag_extended_source(w + d, rho, u1=u1, s=s, q=q, **kw).cpu().numpy()
-    fdx = (fw(h) - fw(-h)) / (2 * h)
-    fdy = (fw(1j * h) - fw(-1j * h)) / (2 * h)
-    ex = np.abs(g[:, 0] - fdx) / np.maximum(np.abs(fdx), 1e-3 * mag.cpu().numpy())
-    ey = np.abs(g[:, 1] - fdy) / np.maximum(np.abs(fdy), 1e-3 * mag.cpu().numpy())
-    assert np.median(ex) < 1e-3 and np.median(ey) < 1e-3
+        h = 1e-9
+        fd = (f(name, h) - f(name, -h)) / (2 * h)
+        fd2 = (f(name, 2 * h) - f(name, -2 * h)) / (4 * h)
+        stable = np.abs(fd - fd2) <= 1e-4 * np.maximum(np.abs(fd), 1e-3 * mag)
+        err = np.abs(g[:, col] - fd) / np.maximum(np.abs(fd), 1e-3 * mag)
+        print(f"{name}: stable {stable.mean():.2f}, max rel err vs FD {err[stable].max():.2e}")
+        assert stable.mean() >= 0.75
+        assert np.all(err[stable] < 1e-3)
+
+
+def test_extended_grad_reference_case(gpu):
+    """The reference's own gradient test (tests/test_extended_source.py:294-331): the
+    first caustic point of s=0.9, q=0.2 (50-point caustic), rho=1e-2, u1=0.7, 300 limb
+    points, 100 LD nodes; jacobian in (s, q, rho, u1) against one-sided differences with
+    epsilon 1e-9, rtol 1e-3."""
+    from scipy.optimize import approx_fprime
+
+    from caustics_amd import critical_and_caustic_curves
+    from caustics_amd.extended_source import extended_fwd_vjp
+
+    s, q, rho, u1 = 0.9, 0.2, 1e-2, 0.7
+    _, ca = critical_and_caustic_curves(npts=50, nlenses=2, s=s, q=q)
+    w0 = complex(ca.reshape(-1)[0].item())
+    kw = dict(nlenses=2, npts_limb=300, limb_darkening=True, npts_ld=100)
+
+    def fn(x):
+        return mag_extended_source(np.array([w0]), x[2], u1=x[3], s=x[0], q=x[1],
+                                   **kw).cpu().numpy()[0]
+
+    x0 = np.array([s, q, rho, u1])
+    jfd = approx_fprime(x0, fn, 1e-9)
+    _, g, _ = extended_fwd_vjp(np.array([w0]), rho, np.ones(1), u1=u1, s=s, q=q, **kw)
+    g = g.cpu().numpy()[0]
+    jac = np.array([g[4], g[5], g[2], g[3]])
+    err = np.abs((jac - jfd) / jfd)
+    print("reference gradient case: rel err", err)
+    assert np.all(err < 1e-3)
 
 
 def test_extended_autograd_matches_fused(gpu):

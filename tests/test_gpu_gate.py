@@ -43,3 +43,108 @@ def test_hexadecapole_u1_and_single_lens(gpu):
         np.testing.assert_allclose(dmu.cpu().numpy(), dmo, rtol=1e-9)
     mu, _, use = hexadecapole_gate(np.array([0.5, 0.01j]), 0.02, nlenses=1)
     assert use.cpu().numpy().tolist() == [True, False]
+
+
+def _orc_mu_hex(nl, w, rho, u1, prm):
+    lp, x_cm = lens.lens_params(nl, **prm)
+    z, mask = lens.images_point_source(np.asarray(w) + x_cm, nl, **lp)
+    return multipole.mag_hexadecapole(z, mask, rho, u1, nl, **lp)[0]
+
+
+_HEX_CASES = {
+    1: {},
+    2: dict(s=1.1, q=0.4),
+    3: dict(s=1.2, q=0.3, q3=0.05, r3=0.9, psi=1.1),
+}
+
+
+@pytest.mark.parametrize("nl", [1, 2, 3])
+def test_hexadecapole_vjp_matches_finite_differences(gpu, nl):
+    """g_out = g * d mu_hex / d(Re w, Im w, rho, u1, params) against central differences
+    of the oracle's hexadecapole (h = 1e-6 relative), on points away from caustics
+    where the image count does not change within the step."""
+    from caustics_amd.lightcurve import hexadecapole_vjp
+
+    prm = _HEX_CASES[nl]
+    g = np.random.default_rng(11 + nl)
+    n = 200
+    r = g.uniform(0.4, 1.6, n)
+    w = r * np.exp(1j * g.uniform(-np.pi, np.pi, n))
+    rho, u1 = 0.03, 0.45
+    gm = g.uniform(0.5, 2.0, n)
+    mu, go = hexadecapole_vjp(w, rho, gm, nlenses=nl, u1=u1, **prm)
+    mu, go = mu.cpu().numpy(), go.cpu().numpy()
+    mo = _orc_mu_hex(nl, w, rho, u1, prm)
+    np.testing.assert_allclose(mu, mo, rtol=1e-10)
+    names = ["w_re", "w_im", "rho", "u1"] + list(prm)
+    base = dict(w=w, rho=rho, u1=u1, **prm)
+
+    def f(key, h):
+        a = dict(base)
+        p = dict(prm)
+        if key == "w_re":
+            a["w"] = w + h
+        elif key == "w_im":
+            a["w"] = w + 1j * h
+        elif key in prm:
+            p[key] = prm[key] + h
+        else:
+            a[key] = base[key] + h
+        return _orc_mu_hex(nl, a["w"], a["rho"], a["u1"], p)
+
+    bad = {}
+    for k, key in enumerate(names):
+        h = 1e-6 * max(abs(base[key]) if key in base and np.ndim(base[key]) == 0 else 1.0, 1e-2)
+        fd = (f(key, h) - f(key, -h)) / (2 * h)
+        fd2 = (f(key, 2 * h) - f(key, -2 * h)) / (4 * h)
+        stable = np.abs(fd - fd2) <= 1e-6 * np.maximum(np.abs(fd), mo)
+        e = np.abs(gm * fd - go[:, k]) / np.maximum(np.abs(gm * fd), 1e-3 * mo)
+        bad[key] = (stable.mean(), e[stable].max())
+        assert stable.mean() > 0.9, (key, stable.mean())
+        assert e[stable].max() < 1e-5, (key, e[stable].max())
+    print(f"nl={nl}: " + ", ".join(f"{k}: stable {v[0]:.2f} max rel {v[1]:.1e}"
+                                   for k, v in bad.items()))
+
+
+def test_mag_autograd_matches_fused_vjp(gpu):
+    """mag() gradients (autograd) equal the fused forward+VJP, which routes accepted
+    points through the hexadecapole VJP and the others through the contour VJP."""
+    import torch
+
+    from caustics_amd import mag, mag_fwd_vjp
+    from caustics_amd.extended_source import extended_fwd_vjp
+    from caustics_amd.lightcurve import hexadecapole_vjp
+
+    s, q, rho, u1 = 0.95, 0.05, 0.02, 0.5
+    _, ca = lens.critical_and_caustic_curves(npts=40, nlenses=2, s=s, q=q)
+    g = np.random.default_rng(3)
+    near = ca.reshape(-1)[::4] + 0.5 * rho * np.exp(1j * g.uniform(0, 2 * np.pi, ca.size // 4))
+    far = 1.3 * np.exp(1j * g.uniform(0, 2 * np.pi, 24))
+    w = np.concatenate([near, far])
+    gm = g.uniform(0.5, 2.0, w.size)
+    kw = dict(nlenses=2, npts_limb=300, limb_darkening=True, u1=u1, s=s, q=q)
+    m, go, use = mag_fwd_vjp(w, rho, gm, **kw)
+    use = use.cpu().numpy()
+    assert 0 < use.sum() < use.size
+    wt = torch.as_tensor(w, device="cuda").requires_grad_(True)
+    rt = torch.full((w.size,), rho, dtype=torch.float64, device="cuda", requires_grad=True)
+    st = torch.full((w.size,), s, dtype=torch.float64, device="cuda", requires_grad=True)
+    qt = torch.full((w.size,), q, dtype=torch.float64, device="cuda", requires_grad=True)
+    ma = mag(wt, rt, nlenses=2, npts_limb=300, limb_darkening=True, u1=u1, s=st, q=qt)
+    (ma * torch.as_tensor(gm, device="cuda")).sum().backward()
+    go = go.cpu().numpy()
+    np.testing.assert_array_equal(ma.detach().cpu().numpy(), m.cpu().numpy())
+    np.testing.assert_allclose(wt.grad.real.cpu().numpy(), go[:, 0], rtol=1e-14, atol=0)
+    np.testing.assert_allclose(wt.grad.imag.cpu().numpy(), go[:, 1], rtol=1e-14, atol=0)
+    np.testing.assert_allclose(rt.grad.cpu().numpy(), go[:, 2], rtol=1e-14, atol=0)
+    np.testing.assert_allclose(st.grad.cpu().numpy(), go[:, 4], rtol=1e-14, atol=0)
+    np.testing.assert_allclose(qt.grad.cpu().numpy(), go[:, 5], rtol=1e-14, atol=0)
+    acc, rej = np.nonzero(use)[0], np.nonzero(~use)[0]
+    _, gh = hexadecapole_vjp(w[acc], rho, gm[acc], nlenses=2, u1=0.0, s=s, q=q)
+    gh = gh.cpu().numpy()
+    np.testing.assert_array_equal(go[acc][:, [0, 1, 2, 4, 5]], gh[:, [0, 1, 2, 4, 5]])
+    assert np.all(go[acc][:, 3] == 0.0)
+    me, ge, _ = extended_fwd_vjp(w[rej], rho, gm[rej], npts_limb=300, limb_darkening=True,
+                                 u1=u1, nlenses=2, s=s, q=q)
+    np.testing.assert_array_equal(m.cpu().numpy()[rej], me.cpu().numpy())
+    np.testing.assert_array_equal(go[rej], ge.cpu().numpy())

@@ -13,9 +13,9 @@ import caustics_amd as ca
 from caustics_amd import workloads as W
 from caustics_amd.lightcurve import lightcurve_fwd_vjp, mag_point_source_lightcurve
 from caustics_amd.point_source import mag_point_source
-from oracle import lens
+from oracle import lens, mp_lens
 
-from helpers import agree_within_spread
+from helpers import agree_with_truth
 
 pytestmark = pytest.mark.gpu
 
@@ -75,9 +75,16 @@ def test_triple_matches_oracle(gpu):
     mo = orc(0.0)
     pert = [orc(4e-16, 1), orc(4e-16, 7)]
     e = rel(m, mo)
-    ok = agree_within_spread(m, mo, pert)
+
+    def truth(idx):
+        out = [mp_lens.mag_point_source_mp(w[i], 3, **{k: p[k][i] for k in
+                                                       ("s", "q", "q3", "r3", "psi")})
+               for i in idx]
+        return np.array([o[0][0] for o in out]), np.array([o[1][0] for o in out])
+
+    ok = agree_with_truth(m, [mo] + pert, truth)
     print(f"triple: frac within 1e-10 of the FP64 oracle {np.mean(e <= 1e-10):.4f}; "
-          f"within the oracle's own 1-ulp coefficient spread {ok.mean():.4f}")
+          f"passing the 40-digit check where not {ok.mean():.4f}")
     assert ok.mean() >= 0.999
 
 
@@ -112,6 +119,7 @@ def test_gradients_vs_oracle_and_fd(gpu, nlenses):
     h = 1e-6
     for k, key in ((2, "s"), (0, "w")):
         fd = np.zeros(n)
+        fd2 = np.zeros(n)
         for i in range(n):
             pi = {kk: p[kk][i] for kk in names}
             if key == "w":
@@ -124,7 +132,21 @@ def test_gradients_vs_oracle_and_fd(gpu, nlenses):
                 mp = lens.mag_point_source(w[i], nlenses=nlenses, **pp)
                 mm = lens.mag_point_source(w[i], nlenses=nlenses, **pm)
             fd[i] = (mp - mm).item() / (2 * h)
-        e = (np.abs(fd - ours[:, k]) / np.maximum(np.abs(fd), 1e-2 * mo))[well]
+            if key == "w":
+                mp2 = lens.mag_point_source(w[i] + 2 * h, nlenses=nlenses, **pi)
+                mm2 = lens.mag_point_source(w[i] - 2 * h, nlenses=nlenses, **pi)
+            else:
+                pp[key] += h
+                pm[key] -= h
+                mp2 = lens.mag_point_source(w[i], nlenses=nlenses, **pp)
+                mm2 = lens.mag_point_source(w[i], nlenses=nlenses, **pm)
+            fd2[i] = (mp2 - mm2).item() / (4 * h)
+        # points where the image count or rounding spoils the difference quotient
+        # (h and 2h disagree) say nothing about the derivative
+        stable = np.abs(fd - fd2) <= 1e-5 * np.maximum(np.abs(fd), mo)
+        e = (np.abs(fd - ours[:, k]) / np.maximum(np.abs(fd), 1e-2 * mo))[well & stable]
+        print(f"FD {key}: stable {np.mean(stable[well]):.3f}, p99 rel err {np.quantile(e, 0.99):.2e}")
+        assert np.mean(stable[well]) > 0.9
         assert np.quantile(e, 0.99) < 1e-4
 
 
@@ -143,9 +165,15 @@ def test_lightcurve_fwd_and_vjp(gpu):
     pert = [np.stack([lens.mag_point_source(wtraj[l], nlenses=2, s=prm["s"][l], q=prm["q"][l],
                                             perturb=4e-16, seed=sd + l) for l in range(L)])
             for sd in (1, 1000)]
-    ok = agree_within_spread(mag, mo, pert)
+    def truth(idx):
+        li, ti = np.unravel_index(idx, mo.shape)
+        out = [mp_lens.mag_point_source_mp(wtraj[l, k], 2, s=prm["s"][l], q=prm["q"][l])
+               for l, k in zip(li, ti)]
+        return np.array([o[0][0] for o in out]), np.array([o[1][0] for o in out])
+
+    ok = agree_with_truth(mag, [mo] + pert, truth)
     print(f"light curves: within 1e-10 {np.mean(rel(mag, mo) <= 1e-10):.5f}, "
-          f"within the oracle's coefficient spread {ok.mean():.5f}")
+          f"passing the 40-digit check where not {ok.mean():.5f}")
     assert ok.mean() > 0.9999
     # autograd path gives the same VJP
     cols = {k: torch.as_tensor(prm[k], device="cuda").requires_grad_(True)
