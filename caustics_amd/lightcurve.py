@@ -245,4 +245,5 @@ def mag_fwd_vjp(w_points, rho, g_mag, nlenses=2, npts_limb=200, limb_darkening=F
     from .extended_source import _col
 
     prm = torch.stack(ps, -1).contiguous() if ps else None
-    return _mag_fwd(cfg, w, rho_t, u1_t, prm, _col(g_mag, w.numel()))
+    m, use, go = _mag_fwd(cfg, w, rho_t, u1_t, prm, _col(g_mag, w.numel()))
+    return m, go, use

@@ -197,8 +197,18 @@ def images_point_source(w, nlenses=2, itmax=2500, compensated=False, init_mode=0
 
 
 def mag_point_source(w, nlenses=2, itmax=2500, compensated=False, init_mode=0, perturb=0.0,
-                     seed=0, **params):
-    """Point-source magnification (point_source.py:1762-1830)."""
+                     seed=0, perturb_inputs=False, **params):
+    """Point-source magnification (point_source.py:1762-1830).
+
+    perturb_inputs: multiply w and every user parameter by (1 + u 2^-52), u uniform in
+    [-1, 1] (seeded) before evaluating -- the 1-ulp input uncertainty every FP64
+    implementation starts from."""
+    if perturb_inputs:
+        g = np.random.default_rng(seed)
+        w = np.asarray(w, complex)
+        w = w * (1 + 2.0**-52 * (g.uniform(-1, 1, w.shape) + 1j * g.uniform(-1, 1, w.shape)))
+        params = {k: v * (1 + 2.0**-52 * g.uniform(-1, 1, np.shape(v)))
+                  for k, v in params.items()}
     lp, x_cm = lens_params(nlenses, **params)
     w = np.asarray(w, complex) + x_cm
     z, mask = images_point_source(w, nlenses, itmax=itmax, compensated=compensated,

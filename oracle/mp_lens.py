@@ -1,23 +1,21 @@
 """High-precision (mpmath) evaluation of the point-source route, for condition-aware
 parity.  TEST INFRASTRUCTURE ONLY.
 
-The FP64 polynomial route (coefficients, roots, image filter, sum of 1/|det J|) loses
-digits near caustics, where images are nearly double roots: both the GPU kernels and
-the FP64 oracle then carry errors far above 1e-10 that differ from each other.  What
-the mathematics fixes is the value for the exact inputs; this module computes it in
-40-digit arithmetic from the FP64 lens-frame inputs (w + x_cm, lens positions and
-masses exactly as FP64 numbers), so that a test can ask whether a result is as close
-to it as the FP64 oracle is.
+The FP64 polynomial route (lens frame, coefficients, roots, image filter, sum of
+1/|det J|) loses digits near caustics, where images are nearly double roots: both the
+GPU kernels and the FP64 oracle then carry errors far above 1e-10 that differ from
+each other.  What the mathematics fixes is the value for the exact inputs; this module
+computes it in 40-digit arithmetic with every input (source position and user lens
+parameters, as given in FP64) taken as exact, so that a test can ask whether a result
+is as close to it as FP64 evaluations of the oracle are.
 
-Same formulas as the oracle: lens polynomial in product form (oracle/lens.py:
-poly_coeffs_product), image filter |lens_eq(z) - w| < 1e-6
-(src/caustics/point_source.py:1703-1707), magnification sum 1/|det J| over real images
-(point_source.py:1828-1830), hexadecapole terms (oracle/multipole.py).
+Same formulas as the oracle: parameter mapping of point_source.py:1798-1817, lens
+polynomial in product form (oracle/lens.py: poly_coeffs_product), image filter
+|lens_eq(z) - w| < 1e-6 (point_source.py:1703-1707), magnification sum 1/|det J| over
+real images (point_source.py:1828-1830).
 """
 import mpmath as mp
 import numpy as np
-
-from . import lens
 
 DPS = 40
 
@@ -35,11 +33,6 @@ def _padd(a, b):
     a = [mp.mpc(0)] * (n - len(a)) + list(a)
     b = [mp.mpc(0)] * (n - len(b)) + list(b)
     return [x + y for x, y in zip(a, b)]
-
-
-def _frame(nlenses, lp):
-    r, e = lens.lens_positions_masses(nlenses, **lp)
-    return [mp.mpc(complex(x)) for x in r], [mp.mpf(float(x)) for x in e]
 
 
 def poly_mp(wl, r, e):
@@ -71,41 +64,56 @@ def poly_mp(wl, r, e):
     return _padd(P, [-c for c in _pmul(D, S)])
 
 
-def images_mp(wl, nlenses, lp):
-    """(roots, lens-equation residuals) at DPS digits for one lens-frame source point."""
-    with mp.workdps(DPS):
-        r, e = _frame(nlenses, lp)
-        w = mp.mpc(complex(wl))
-        c = poly_mp(w, r, e)
-        z = mp.polyroots(c, maxsteps=400, extraprec=4 * DPS)
-        res = []
-        for zi in z:
-            zb = mp.conj(zi)
-            wz = zi - sum(ej / (zb - mp.conj(rj)) for ej, rj in zip(e, r))
-            res.append(abs(wz - w))
-        return z, res, r, e
+def frame_mp(nlenses, s=None, q=None, q3=None, r3=None, psi=None):
+    """Lens positions, masses and x_cm at DPS digits from the user parameters taken as
+    exact (the mapping of point_source.py:1798-1817)."""
+    s, q = mp.mpf(float(s)), mp.mpf(float(q))
+    a = s / 2
+    x_cm = a * (1 - q) / (1 + q)
+    if nlenses == 2:
+        e1 = 1 / (1 + q)
+        return [mp.mpc(a), mp.mpc(-a)], [e1, 1 - e1], x_cm
+    q3 = mp.mpf(float(q3))
+    e1 = q / (1 + q + q3)
+    e2 = q * e1
+    r3c = mp.mpf(float(r3)) * mp.expj(mp.mpf(float(psi)))
+    return [mp.mpc(a), mp.mpc(-a), r3c], [e1, e2, 1 - e1 - e2], x_cm
 
 
 def mag_point_source_mp(w, nlenses, **params):
-    """(magnification, ambiguous) for each source position w (user frame).
-    ambiguous is True where a root's residual lies within a factor 10 of the 1e-6
-    filter, so that rounding can legitimately flip the image count."""
-    lp, x_cm = lens.lens_params(nlenses, **params)
+    """(magnification, undecided) for each source position w (user frame), with every
+    input (w and the user lens parameters) taken as exact.
+
+    undecided is the part of the magnification that an FP64 evaluation cannot decide:
+    inf where a root's residual lies within a factor 10 of the 1e-6 filter, else the
+    sum of the contributions of real images that sit in a root cluster (another root
+    closer than 1e-3 |z|): FP64 roots there carry errors up to eps^(1/m) of the cluster
+    size, enough to push their residual past the filter."""
     w = np.atleast_1d(np.asarray(w, complex))
     out = np.zeros(w.shape)
-    amb = np.zeros(w.shape, bool)
+    amb = np.zeros(w.shape)
     for i, wi in enumerate(w):
-        wl = complex(wi + x_cm)
         with mp.workdps(DPS):
-            z, res, r, e = images_mp(wl, nlenses, lp)
+            r, e, x_cm = frame_mp(nlenses, **params)
+            wl = mp.mpc(complex(wi)) + x_cm
+            c = poly_mp(wl, r, e)
+            z = mp.polyroots(c, maxsteps=400, extraprec=4 * DPS)
+            res = []
+            for zi in z:
+                zb = mp.conj(zi)
+                res.append(abs(zi - sum(ej / (zb - mp.conj(rj)) for ej, rj in zip(e, r)) - wl))
             m = mp.mpf(0)
-            for zi, ri in zip(z, res):
+            for k, (zi, ri) in enumerate(zip(z, res)):
                 if 1e-7 < ri < 1e-5:
-                    amb[i] = True
+                    amb[i] = np.inf
                 if ri < 1e-6:
                     zb = mp.conj(zi)
                     K = sum(ej / (zb - mp.conj(rj)) ** 2 for ej, rj in zip(e, r))
-                    m += 1 / abs(1 - abs(K) ** 2)
+                    c = 1 / abs(1 - abs(K) ** 2)
+                    m += c
+                    dmin = min(abs(zi - zj) for j, zj in enumerate(z) if j != k)
+                    if dmin < 1e-3 * max(1, abs(zi)):
+                        amb[i] += float(c)
             out[i] = float(m)
     return out, amb
 

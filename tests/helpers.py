@@ -64,11 +64,11 @@ def roots_close(z_test, z_ref, coeffs_asc, rel=1e-12, kappa_factor=64.0, truth=T
 
 def agree_with_truth(test, oracle_runs, truth_fn, rel=1e-10, factor=10.0):
     """Per item: |test - oracle| <= rel |oracle|; where that fails the item is judged
-    against the 40-digit value truth_fn(indices) -> (truth, ambiguous): it passes if
-    ambiguous (an image within a factor 10 of the 1e-6 filter) or if
-    |test - truth| <= max(rel |truth|, factor * max_k |oracle_k - truth|), the oracle
-    runs being the plain FP64 oracle and copies with 1-ulp coefficient perturbations
-    (the size of FP64 error the route itself has at that point)."""
+    against the 40-digit value truth_fn(indices) -> (truth, undecided): it passes if
+    |test - truth| <= max(rel |truth|, factor * max_k |oracle_k - truth|, undecided),
+    the oracle runs being the plain FP64 oracle and copies with 1-ulp input
+    perturbations (the size of FP64 error the route itself has at that point), and
+    `undecided` the part of the value FP64 cannot decide (clustered images)."""
     test = np.asarray(test, float)
     runs = [np.asarray(r, float) for r in oracle_runs]
     ok = np.abs(test - runs[0]) <= rel * np.abs(runs[0])
@@ -77,7 +77,8 @@ def agree_with_truth(test, oracle_runs, truth_fn, rel=1e-10, factor=10.0):
         tr, amb = truth_fn(idx)
         tf, rf = test.reshape(-1)[idx], [r.reshape(-1)[idx] for r in runs]
         scale = np.max([np.abs(r - tr) for r in rf], axis=0)
-        good = amb | (np.abs(tf - tr) <= np.maximum(rel * np.abs(tr), factor * scale))
+        tol = np.maximum(np.maximum(rel * np.abs(tr), factor * scale), 1.000001 * amb)
+        good = np.abs(tf - tr) <= tol
         ok.reshape(-1)[idx] = good
     return ok
 

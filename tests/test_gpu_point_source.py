@@ -70,10 +70,10 @@ def test_triple_matches_oracle(gpu):
     def orc(perturb, seed=0):
         return np.array([lens.mag_point_source(w[i], nlenses=3, s=p["s"][i], q=p["q"][i],
                                                q3=p["q3"][i], r3=p["r3"][i], psi=p["psi"][i],
-                                               perturb=perturb, seed=seed + i).item()
+                                               perturb_inputs=seed > 0, seed=seed + i).item()
                          for i in range(3000)])
     mo = orc(0.0)
-    pert = [orc(4e-16, 1), orc(4e-16, 7)]
+    pert = [orc(0.0, 1), orc(0.0, 7)]
     e = rel(m, mo)
 
     def truth(idx):
@@ -163,7 +163,7 @@ def test_lightcurve_fwd_and_vjp(gpu):
     mo = np.stack([lens.mag_point_source(wtraj[l], nlenses=2, s=prm["s"][l], q=prm["q"][l])
                    for l in range(L)])
     pert = [np.stack([lens.mag_point_source(wtraj[l], nlenses=2, s=prm["s"][l], q=prm["q"][l],
-                                            perturb=4e-16, seed=sd + l) for l in range(L)])
+                                            perturb_inputs=True, seed=sd + l) for l in range(L)])
             for sd in (1, 1000)]
     def truth(idx):
         li, ti = np.unravel_index(idx, mo.shape)
