@@ -13,8 +13,25 @@ extern "C" int64_t caus_extended_workspace_bytes(int32_t nlenses, int64_t S, int
     if (nlenses < 1 || nlenses > 3 || S < 0 || npts_max < 4 || npts_max > 2 * kExtSortMax)
         return -1;
     const int deg = nlenses == 1 ? 2 : nlenses * nlenses + 1;
-    return (int64_t)ext_ws_bytes(deg, ext_mmax(npts_max)) * S;
+    return (int64_t)ext_total_bytes(deg, S, npts_max);
 }
+
+namespace caus {
+// grid of the flat extended-source kernels: enough lanes for the work list, at most a
+// few resident waves per SIMD (the kernels stride over their work list)
+int ext_flat_grid(int64_t max_lanes) {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        int c = 0;
+        if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            c > 0)
+            cus = c;
+    }
+    const int64_t need = (max_lanes + kExtFlatBlock - 1) / kExtFlatBlock;
+    const int64_t cap = (int64_t)cus * 8;
+    return (int)(need < 1 ? 1 : (need < cap ? need : cap));
+}
+}  // namespace caus
 
 extern "C" int caus_mag_extended_source(void* stream, int32_t nlenses, const double* w0, int64_t S,
                                         const double* rho, const double* params,
@@ -46,8 +63,8 @@ extern "C" int caus_mag_extended_source_vjp(void* stream, int32_t nlenses, const
     if (!w0 || !rho || !mag || !workspace || (nlenses > 1 && !params)) return CAUS_ERR_ARG;
     if (S > 0x7fffffff) return CAUS_ERR_ARG;
     const int deg = nlenses == 1 ? 2 : nlenses * nlenses + 1;
-    const size_t per = ext_ws_bytes(deg, ext_mmax(npts_max));
-    if (workspace_bytes < (int64_t)(per * (size_t)S)) return CAUS_ERR_ARG;
+    const size_t per = ext_source_bytes(deg, ext_mmax(npts_max));
+    if (workspace_bytes < (int64_t)ext_total_bytes(deg, S, npts_max)) return CAUS_ERR_ARG;
     ExtArgs a;
     a.w0 = reinterpret_cast<const cd*>(w0);
     a.S = S;
@@ -68,6 +85,7 @@ extern "C" int caus_mag_extended_source_vjp(void* stream, int32_t nlenses, const
     a.ws = reinterpret_cast<char*>(workspace);
     a.ws_per_source = per;
     a.mmax = ext_mmax(npts_max);
+    a.nadd_max = ext_n_add(npts_max);
     a.itmax = itmax;
     hipStream_t s = (hipStream_t)stream;
     switch (nlenses) {

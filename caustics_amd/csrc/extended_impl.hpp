@@ -148,10 +148,13 @@ __device__ __forceinline__ uint64_t nn_map(const cd (&a)[DEG], const cd (&b)[DEG
 }
 
 // ---- point solve -----------------------------------------------------------
+// warm: z holds a nearby point's images on entry and starts the iteration (a limb
+// neighbour, as the reference's sequential solver does); a warm start that fails to
+// converge is retried from the standard initial estimates.
 template <int NL, int MODE>
 __device__ __forceinline__ bool ext_solve_point(cd w, const LensFrame<NL>& lf,
                                                 cd (&z)[ext_degree<NL>()], uint32_t& flags,
-                                                int itmax) {
+                                                int itmax, bool warm = false) {
     constexpr int DEG = ext_degree<NL>();
     bool ok = true;
     flags = 0;
@@ -166,7 +169,13 @@ __device__ __forceinline__ bool ext_solve_point(cd w, const LensFrame<NL>& lf,
             if (norm2(z[j]) < 1.0) flags |= 1u << (16 + j);
         }
     } else {
-        ok = solve_lens<NL, MODE>(w, lf, z, itmax);
+        ok = solve_lens<NL, MODE>(w, lf, z, itmax, warm);
+        if (warm) {
+            bool fin = true;
+#pragma unroll
+            for (int j = 0; j < DEG; ++j) fin = fin && isfinite(z[j].re) && isfinite(z[j].im);
+            if (!ok || !fin) ok = solve_lens<NL, MODE>(w, lf, z, itmax, false);
+        }
 #pragma unroll
         for (int j = 0; j < DEG; ++j) {
             ImageTerms<NL> it = image_terms<NL>(z[j], lf);
@@ -834,7 +843,27 @@ __device__ void contour_adjoint(const cd* zb, const double* theta, const Piece* 
 
 }  // namespace caus
 
+// ============================================================================
+// Batch pipeline.  The contours of a batch of sources are computed by a fixed
+// sequence of launches over the whole batch, so that the root solves (nearly all of
+// the arithmetic) run one limb point per lane at the solver's occupancy:
+//   k_ext_plan        1 block      effective npts, prefix offsets of the flat work lists
+//   k_ext_limb_init   flat         initial limb angles, kExtPer consecutive angles per
+//                                  lane, each warm-started from the previous one
+//   k_ext_chain       per source   image continuation (extended_source.py:34-53)
+//   10 x k_ext_select per source   gap sort, insertion shifts, move old points, new tasks
+//        k_ext_solve_add flat      new points warm-started from their left neighbour
+//   k_ext_chain       per source
+//   k_ext_contours    per source   closure test, segments, stitching -> piece lists
+//   k_ext_integrate   per source   contour integrals, magnification, reverse mode
+// No launch allocates, synchronises with the host or clears memory with a memset, so
+// the sequence can be captured in a HIP graph.
+// ============================================================================
 namespace caus {
+
+constexpr int kExtPer = 4;          // initial limb angles per lane of k_ext_limb_init
+constexpr int kExtFlatBlock = 256;  // block of the flat kernels
+constexpr int kExtPlanBlock = 1024;
 
 struct ExtArgs {
     const cd* w0;
@@ -851,228 +880,363 @@ struct ExtArgs {
     uint8_t* status;
     const double* g_mag;   // VJP only
     double* g_out;         // VJP only: (S, 4 + P)
-    char* ws;
+    char* ws;              // batch header, then one block of ws_per_source bytes per source
     size_t ws_per_source;
     int mmax;
+    int nadd_max;
     int itmax;
 };
 
-template <int NL, int MODE, bool LD, bool VJP>
-__global__ __launch_bounds__(kExtBlock) void k_mag_extended(ExtArgs A) {
-    constexpr int DEG = ext_degree<NL>();
-    constexpr int ND = ext_nd<NL>();
-    constexpr int NIMG = NL * NL + 1;
-    constexpr int NSEG = 3 * NIMG;   // nr_of_segments
-    __shared__ uint64_t s_key[kExtSortMax];
-    __shared__ int s_idx[kExtSortMax];
-    __shared__ int s_int[kExtBlock];
-    __shared__ double s_d[8];
-    __shared__ double s_gl[4 * 128];
-    __shared__ int s_fail;
-    __shared__ Seg s_seg[kExtMaxSeg];
-    __shared__ int s_nseg;
-    __shared__ ActSeg s_con[3];
-    __shared__ unsigned s_allmask;
+// contours of one source, written by k_ext_contours
+struct ExtContours {
+    unsigned closed;  // image rows that are closed contours by themselves
+    int open;         // 1 when the three stitched contours in con[] are used
+    ActSeg con[3];
+};
 
-    const int tid = threadIdx.x;
-    const int64_t src = blockIdx.x;
-    if (src >= A.S) return;
-    const double* prm = A.params + src * A.pstride;
-    const double rho = A.rho[src];
-    const double u1 = LD ? A.u1[src * A.u1stride] : 0.0;
-    LensFrame<NL> lf;
+// batch header of the workspace
+struct ExtBatch {
+    int64_t* off_init;  // S + 1: prefix of the lanes of k_ext_limb_init
+    int64_t* off_add;   // S + 1: prefix of the new points of one refinement round
+    int32_t* npts;      // S: effective limb resolution
+    uint8_t* fail;      // S: a solve did not converge, or npts was out of range
+    int32_t* t_src;     // new-point tasks of the current round (capacity S * nadd_max)
+    int32_t* t_left;
+    int32_t* t_dst;
+    double* t_th;
+};
+
+__host__ __device__ inline size_t ext_task_cap(int64_t S, int nadd_max) {
+    return (size_t)S * (size_t)(nadd_max > 0 ? nadd_max : 1);
+}
+
+__host__ __device__ inline size_t ext_header_bytes(int64_t S, int nadd_max) {
+    const size_t cap = ext_task_cap(S, nadd_max);
+    return ext_align(2 * (size_t)(S + 1) * sizeof(int64_t)) + ext_align((size_t)S * 4) +
+           ext_align((size_t)S) + ext_align(3 * cap * sizeof(int32_t)) +
+           ext_align(cap * sizeof(double));
+}
+
+__host__ __device__ inline size_t ext_source_bytes(int deg, int mmax) {
+    return ext_ws_bytes(deg, mmax) + ext_align(sizeof(ExtContours));
+}
+
+__host__ __device__ inline size_t ext_total_bytes(int deg, int64_t S, int npts_max) {
+    return ext_header_bytes(S, ext_n_add(npts_max)) +
+           (size_t)S * ext_source_bytes(deg, ext_mmax(npts_max));
+}
+
+__device__ inline ExtBatch ext_batch(const ExtArgs& A) {
+    ExtBatch b;
+    char* p = A.ws;
+    const size_t cap = ext_task_cap(A.S, A.nadd_max);
+    b.off_init = (int64_t*)p;
+    b.off_add = b.off_init + (A.S + 1);
+    p += ext_align(2 * (size_t)(A.S + 1) * sizeof(int64_t));
+    b.npts = (int32_t*)p;
+    p += ext_align((size_t)A.S * 4);
+    b.fail = (uint8_t*)p;
+    p += ext_align((size_t)A.S);
+    b.t_src = (int32_t*)p;
+    b.t_left = b.t_src + cap;
+    b.t_dst = b.t_left + cap;
+    p += ext_align(3 * cap * sizeof(int32_t));
+    b.t_th = (double*)p;
+    return b;
+}
+
+template <int DEG>
+__device__ inline char* ext_source_base(const ExtArgs& A, int64_t s) {
+    return A.ws + ext_header_bytes(A.S, A.nadd_max) + (size_t)s * A.ws_per_source;
+}
+template <int DEG>
+__device__ inline ExtContours* ext_contours(const ExtArgs& A, int64_t s) {
+    return (ExtContours*)(ext_source_base<DEG>(A, s) + ext_ws_bytes(DEG, A.mmax));
+}
+
+// lens frame, lens-frame source centre and radius of source s
+template <int NL>
+__device__ inline void ext_source(const ExtArgs& A, int64_t s, LensFrame<NL>& lf, cd& w0,
+                                  double& rho) {
     double x_cm = 0.0;
     if constexpr (NL == 1) {
         lf.e[0] = 1.0;
         lf.r[0] = cd{0.0, 0.0};
     } else {
-        lens_frame<NL>(prm, lf, x_cm);
+        lens_frame<NL>(A.params + s * A.pstride, lf, x_cm);
     }
-    const cd w0 = A.w0[src] + cd{x_cm, 0.0};
-    int npts = A.npts ? A.npts[src] : A.npts_def;
-    bool bad_npts = false;
-    if (npts < 4 || npts > A.npts_def) {  // the workspace is sized for npts_def
-        bad_npts = true;
-        npts = npts < 4 ? 4 : A.npts_def;
-    }
-    const int M0 = ext_npts_init(npts), nadd = ext_n_add(npts);
-    ExtWS W = ext_ws(A.ws + (size_t)src * A.ws_per_source, DEG, A.mmax);
-    if (tid == 0) s_fail = bad_npts ? 1 : 0;
-    GLNodes g{};
-    if constexpr (LD) {
-        g.n1 = A.npts_ld / 2;
-        g.n2 = A.npts_ld - g.n1;
-        g.x1 = s_gl;
-        g.w1 = s_gl + 128;
-        g.x2 = s_gl + 256;
-        g.w2 = s_gl + 384;
-        gauss_legendre(g.n1, s_gl, s_gl + 128);
-        gauss_legendre(g.n2, s_gl + 256, s_gl + 384);
-    }
-    __syncthreads();
+    w0 = A.w0[s] + cd{x_cm, 0.0};
+    rho = A.rho[s];
+}
 
-    // ---- initial limb points (extended_source.py:100-106)
-    const double PI = 3.14159265358979323846;
-    for (int k = tid; k < M0; k += kExtBlock) {
-        const double th = (k < M0 - 1) ? -PI + k * (2.0 * PI / (M0 - 1)) : PI - 1e-8;
-        double sn, cs;
-        sincos(th, &sn, &cs);
+// limb points of source s after `round` refinement rounds
+__device__ inline int ext_points(int npts, int round) {
+    return ext_npts_init(npts) + round * ext_n_add(npts);
+}
+
+constexpr double kExtPi = 3.14159265358979323846;
+
+// ---- plan: effective npts (out-of-range values clamped and flagged) and the prefix
+// offsets of the flat work lists, one block scanning the batch in chunks
+static __global__ __launch_bounds__(kExtPlanBlock) void k_ext_plan(ExtArgs A) {
+    __shared__ int64_t s_a[kExtPlanBlock], s_b[kExtPlanBlock];
+    const ExtBatch B = ext_batch(A);
+    const int tid = threadIdx.x;
+    int64_t carry_a = 0, carry_b = 0;
+    for (int64_t base = 0; base < A.S; base += kExtPlanBlock) {
+        const int64_t s = base + tid;
+        int64_t ca = 0, cb = 0;
+        if (s < A.S) {
+            int n = A.npts ? A.npts[s] : A.npts_def;
+            bool bad = false;
+            if (n < 4 || n > A.npts_def) {
+                bad = true;
+                n = n < 4 ? 4 : A.npts_def;
+            }
+            B.npts[s] = n;
+            B.fail[s] = bad ? 1 : 0;
+            ca = (ext_npts_init(n) + kExtPer - 1) / kExtPer;
+            cb = ext_n_add(n);
+        }
+        s_a[tid] = ca;
+        s_b[tid] = cb;
+        __syncthreads();
+        for (int off = 1; off < kExtPlanBlock; off <<= 1) {
+            const int64_t ta = tid >= off ? s_a[tid - off] : 0;
+            const int64_t tb = tid >= off ? s_b[tid - off] : 0;
+            __syncthreads();
+            s_a[tid] += ta;
+            s_b[tid] += tb;
+            __syncthreads();
+        }
+        if (s < A.S) {
+            B.off_init[s] = carry_a + s_a[tid] - ca;
+            B.off_add[s] = carry_b + s_b[tid] - cb;
+        }
+        carry_a += s_a[kExtPlanBlock - 1];
+        carry_b += s_b[kExtPlanBlock - 1];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        B.off_init[A.S] = carry_a;
+        B.off_add[A.S] = carry_b;
+    }
+}
+
+// ---- initial limb points (extended_source.py:100-106): one run of kExtPer consecutive
+// angles per lane, the first from the standard initial estimates, the others from the
+// previous angle's images
+template <int NL, int MODE>
+__global__ __launch_bounds__(kExtFlatBlock) void k_ext_limb_init(ExtArgs A) {
+    constexpr int DEG = ext_degree<NL>();
+    const ExtBatch B = ext_batch(A);
+    const int64_t total = B.off_init[A.S];
+    const int64_t stride = (int64_t)gridDim.x * kExtFlatBlock;
+    for (int64_t t = (int64_t)blockIdx.x * kExtFlatBlock + threadIdx.x; t < total; t += stride) {
+        int64_t lo = 0, hi = A.S;  // off_init[lo] <= t < off_init[hi]
+        while (hi - lo > 1) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (B.off_init[mid] <= t) lo = mid;
+            else hi = mid;
+        }
+        const int64_t s = lo;
+        const int M0 = ext_npts_init(B.npts[s]);
+        const int k0 = (int)(t - B.off_init[s]) * kExtPer, k1 = min(M0, k0 + kExtPer);
+        LensFrame<NL> lf;
+        cd w0;
+        double rho;
+        ext_source<NL>(A, s, lf, w0, rho);
+        ExtWS W = ext_ws(ext_source_base<DEG>(A, s), DEG, A.mmax);
         cd z[DEG];
-        uint32_t f;
-        if (!ext_solve_point<NL, MODE>(w0 + rho * cd{cs, sn}, lf, z, f, A.itmax)) s_fail = 1;
-        W.theta[0][k] = th;
-        store_pt<DEG>(W.z[0], k, z);
-        W.flg[0][k] = f;
-    }
-    __syncthreads();
-    chain_images<DEG>(W.z[0], W.flg[0], W.perm, M0);
-
-    // ---- refinement rounds (extended_source.py:108-135)
-    int cur = 0, M = M0;
-    for (int round = 0; round < 10; ++round) {
-        const cd* zb = W.z[cur];
-        const uint32_t* fb = W.flg[cur];
-        int P2 = 1;
-        while (P2 < M - 1) P2 <<= 1;
-        for (int k = tid; k < P2; k += kExtBlock) {
-            uint64_t key = 0;
-            int idx = -1;
-            if (k < M - 1) {
-                cd a[DEG], b[DEG];
-                load_pt<DEG>(zb, k, a);
-                load_pt<DEG>(zb, k + 1, b);
-                const uint32_t m = (fb[k] | fb[k + 1]) & 0xFFFFu;
-                double dmax = 0.0;
-#pragma unroll
-                for (int r = 0; r < DEG; ++r)
-                    if ((m >> r) & 1u) dmax = fmax(dmax, sqrt(norm2(b[r] - a[r])));
-                key = (uint64_t)__double_as_longlong(dmax);
-                idx = k;
-            }
-            s_key[k] = key;
-            s_idx[k] = idx;
-        }
-        __syncthreads();
-        // bitonic sort, descending by (key, idx)
-        for (int size = 2; size <= P2; size <<= 1) {
-            for (int stride = size >> 1; stride > 0; stride >>= 1) {
-                for (int i = tid; i < P2; i += kExtBlock) {
-                    const int j = i ^ stride;
-                    if (j > i) {
-                        const bool desc = (i & size) == 0;
-                        const uint64_t ki = s_key[i], kj = s_key[j];
-                        const int ii = s_idx[i], ij = s_idx[j];
-                        const bool i_less = (ki < kj) || (ki == kj && ii < ij);
-                        if (desc == i_less) {
-                            s_key[i] = kj;
-                            s_key[j] = ki;
-                            s_idx[i] = ij;
-                            s_idx[j] = ii;
-                        }
-                    }
-                }
-                __syncthreads();
-            }
-        }
-        // mark the selected intervals; exclusive prefix count -> shifts (perm[] as ints)
-        int* shift = reinterpret_cast<int*>(W.perm);
-        const int per = (M + kExtBlock - 1) / kExtBlock;
-        const int k0 = tid * per, k1 = min(M, k0 + per);
-        for (int k = k0; k < k1; ++k) shift[k] = 0;
-        __syncthreads();
-        for (int c = tid; c < nadd; c += kExtBlock) shift[s_idx[c]] = 1;
-        __syncthreads();
-        int cnt = 0;
-        for (int k = k0; k < k1; ++k) cnt += shift[k];
-        int total;
-        int run = block_exclusive_scan(cnt, s_int, total);
         for (int k = k0; k < k1; ++k) {
-            const int ins = shift[k];
-            shift[k] = run;
-            run += ins;
-        }
-        __syncthreads();
-        const int nxt = cur ^ 1;
-        // move old points
-        for (int k = tid; k < M; k += kExtBlock) {
-            const int d = k + shift[k];
-            cd z[DEG];
-            load_pt<DEG>(zb, k, z);
-            store_pt<DEG>(W.z[nxt], d, z);
-            W.flg[nxt][d] = fb[k];
-            W.theta[nxt][d] = W.theta[cur][k];
-        }
-        // new points, matched to their left neighbour
-        for (int c = tid; c < nadd; c += kExtBlock) {
-            const int k = s_idx[c];
-            const double th = 0.5 * (W.theta[cur][k] + W.theta[cur][k + 1]);
+            const double th = (k < M0 - 1) ? -kExtPi + k * (2.0 * kExtPi / (M0 - 1)) : kExtPi - 1e-8;
             double sn, cs;
             sincos(th, &sn, &cs);
-            cd z[DEG], left[DEG];
             uint32_t f;
-            if (!ext_solve_point<NL, MODE>(w0 + rho * cd{cs, sn}, lf, z, f, A.itmax)) s_fail = 1;
-            load_pt<DEG>(zb, k, left);
-            const uint64_t p = greedy_match<DEG>(left, z);
-            permute_pt<DEG>(z, p);
-            f = permute_flags<DEG>(f, p);
-            const int d = k + shift[k] + 1;
-            store_pt<DEG>(W.z[nxt], d, z);
-            W.flg[nxt][d] = f;
-            W.theta[nxt][d] = th;
+            if (!ext_solve_point<NL, MODE>(w0 + rho * cd{cs, sn}, lf, z, f, A.itmax, k > k0))
+                B.fail[s] = 1;
+            W.theta[0][k] = th;
+            store_pt<DEG>(W.z[0], k, z);
+            W.flg[0][k] = f;
         }
-        __syncthreads();
-        cur = nxt;
-        M += nadd;
     }
-    chain_images<DEG>(W.z[cur], W.flg[cur], W.perm, M);
+}
 
+// ---- image continuation over the points of buffer `buf` after `round` rounds
+template <int NL>
+__global__ __launch_bounds__(kExtBlock) void k_ext_chain(ExtArgs A, int buf, int round) {
+    constexpr int DEG = ext_degree<NL>();
+    const int64_t s = blockIdx.x;
+    const ExtBatch B = ext_batch(A);
+    const int M = ext_points(B.npts[s], round);
+    ExtWS W = ext_ws(ext_source_base<DEG>(A, s), DEG, A.mmax);
+    chain_images<DEG>(W.z[buf], W.flg[buf], W.perm, M);
+}
+
+// ---- one refinement round, part 1 (extended_source.py:108-135): the n_add largest
+// image gaps (bitonic sort in LDS, descending by (gap, index)), insertion shifts, old
+// points moved to the other buffer, one task per new point
+template <int NL>
+__global__ __launch_bounds__(kExtBlock) void k_ext_select(ExtArgs A, int cur, int round) {
+    constexpr int DEG = ext_degree<NL>();
+    extern __shared__ uint64_t s_dyn[];
+    __shared__ int s_int[kExtBlock];
+    const int tid = threadIdx.x;
+    const int64_t src = blockIdx.x;
+    const ExtBatch B = ext_batch(A);
+    const int n = B.npts[src];
+    const int M = ext_points(n, round), nadd = ext_n_add(n);
+    ExtWS W = ext_ws(ext_source_base<DEG>(A, src), DEG, A.mmax);
     const cd* zb = W.z[cur];
     const uint32_t* fb = W.flg[cur];
-    const double* thb = W.theta[cur];
-    double total = 0.0;
-    double gacc[ND];
+    int P2 = 1;
+    while (P2 < M - 1) P2 <<= 1;
+    uint64_t* s_key = s_dyn;
+    int* s_idx = reinterpret_cast<int*>(s_dyn + P2);
+    for (int k = tid; k < P2; k += kExtBlock) {
+        uint64_t key = 0;
+        int idx = -1;
+        if (k < M - 1) {
+            cd a[DEG], b[DEG];
+            load_pt<DEG>(zb, k, a);
+            load_pt<DEG>(zb, k + 1, b);
+            const uint32_t m = (fb[k] | fb[k + 1]) & 0xFFFFu;
+            double dmax = 0.0;
 #pragma unroll
-    for (int d = 0; d < ND; ++d) gacc[d] = 0.0;
-    ExtTangents<NL> T;
-    if constexpr (VJP) ext_tangents<NL>(prm, T);
-    auto do_contour = [&](const Piece* pcs, int npc, int L, int nterms, double cdiv, int par) {
-        const double I = integrate_contour<NL, DEG, LD>(zb, pcs, npc, L, nterms, cdiv, W.pq, lf,
-                                                        w0, rho, u1, g, s_d);
-        total += par * I;
-        if constexpr (VJP)
-            contour_adjoint<NL, DEG, LD, ND>(zb, thb, pcs, npc, L, nterms, cdiv, W.pq, lf, w0, rho,
-                                             u1, g, T, (double)par, gacc, s_d);
-    };
-    if constexpr (NL == 1) {
-        // both images, tail index N-1 (extended_source.py:860-866)
-        for (int r = 0; r < 2; ++r) {
-            Piece pc{r, 0, M, 0};
-            const int par = ((fb[0] >> (16 + r)) & 1u) ? -1 : 1;
-            do_contour(&pc, 1, M, M - 1, (double)M, par);
+            for (int r = 0; r < DEG; ++r)
+                if ((m >> r) & 1u) dmax = fmax(dmax, sqrt(norm2(b[r] - a[r])));
+            key = (uint64_t)__double_as_longlong(dmax);
+            idx = k;
         }
-    } else {
-        // closure test (extended_source.py:289-291)
-        if (tid == 0) s_allmask = 0xFFFFu;
-        __syncthreads();
-        unsigned am = 0xFFFFu;
-        for (int k = tid; k < M; k += kExtBlock) am &= fb[k];
-        atomicAnd(&s_allmask, am);
-        __syncthreads();
-        unsigned closed = 0;
+        s_key[k] = key;
+        s_idx[k] = idx;
+    }
+    __syncthreads();
+    for (int size = 2; size <= P2; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = tid; i < P2; i += kExtBlock) {
+                const int j = i ^ stride;
+                if (j > i) {
+                    const bool desc = (i & size) == 0;
+                    const uint64_t ki = s_key[i], kj = s_key[j];
+                    const int ii = s_idx[i], ij = s_idx[j];
+                    const bool i_less = (ki < kj) || (ki == kj && ii < ij);
+                    if (desc == i_less) {
+                        s_key[i] = kj;
+                        s_key[j] = ki;
+                        s_idx[i] = ij;
+                        s_idx[j] = ii;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // selected intervals -> exclusive prefix count = shift of every old point
+    int* shift = reinterpret_cast<int*>(W.perm);
+    const int per = (M + kExtBlock - 1) / kExtBlock;
+    const int k0 = tid * per, k1 = min(M, k0 + per);
+    for (int k = k0; k < k1; ++k) shift[k] = 0;
+    __syncthreads();
+    for (int c = tid; c < nadd; c += kExtBlock) shift[s_idx[c]] = 1;
+    __syncthreads();
+    int cnt = 0;
+    for (int k = k0; k < k1; ++k) cnt += shift[k];
+    int total;
+    int run = block_exclusive_scan(cnt, s_int, total);
+    for (int k = k0; k < k1; ++k) {
+        const int ins = shift[k];
+        shift[k] = run;
+        run += ins;
+    }
+    __syncthreads();
+    const int nxt = cur ^ 1;
+    for (int k = tid; k < M; k += kExtBlock) {
+        const int d = k + shift[k];
+        cd z[DEG];
+        load_pt<DEG>(zb, k, z);
+        store_pt<DEG>(W.z[nxt], d, z);
+        W.flg[nxt][d] = fb[k];
+        W.theta[nxt][d] = W.theta[cur][k];
+    }
+    const int64_t t0 = B.off_add[src];
+    for (int c = tid; c < nadd; c += kExtBlock) {
+        const int k = s_idx[c];
+        B.t_src[t0 + c] = (int32_t)src;
+        B.t_left[t0 + c] = k;
+        B.t_dst[t0 + c] = k + shift[k] + 1;
+        B.t_th[t0 + c] = 0.5 * (W.theta[cur][k] + W.theta[cur][k + 1]);
+    }
+}
+
+// ---- one refinement round, part 2: every new point of the batch, one per lane,
+// started from its left neighbour's images and matched to them (match_points)
+template <int NL, int MODE>
+__global__ __launch_bounds__(kExtFlatBlock) void k_ext_solve_add(ExtArgs A, int cur) {
+    constexpr int DEG = ext_degree<NL>();
+    const ExtBatch B = ext_batch(A);
+    const int64_t total = B.off_add[A.S];
+    const int64_t stride = (int64_t)gridDim.x * kExtFlatBlock;
+    const int nxt = cur ^ 1;
+    for (int64_t t = (int64_t)blockIdx.x * kExtFlatBlock + threadIdx.x; t < total; t += stride) {
+        const int64_t s = B.t_src[t];
+        const int k = B.t_left[t], d = B.t_dst[t];
+        const double th = B.t_th[t];
+        LensFrame<NL> lf;
+        cd w0;
+        double rho;
+        ext_source<NL>(A, s, lf, w0, rho);
+        ExtWS W = ext_ws(ext_source_base<DEG>(A, s), DEG, A.mmax);
+        double sn, cs;
+        sincos(th, &sn, &cs);
+        cd z[DEG], left[DEG];
+        load_pt<DEG>(W.z[cur], k, left);
 #pragma unroll
-        for (int r = 0; r < DEG; ++r) {
-            if ((s_allmask >> r) & 1u) {
-                if (norm2(pt<DEG>(zb, 0, r) - pt<DEG>(zb, M - 1, r)) < 1e-10) closed |= 1u << r;
-            }
+        for (int r = 0; r < DEG; ++r) z[r] = left[r];
+        uint32_t f;
+        if (!ext_solve_point<NL, MODE>(w0 + rho * cd{cs, sn}, lf, z, f, A.itmax, true))
+            B.fail[s] = 1;
+        const uint64_t p = greedy_match<DEG>(left, z);
+        permute_pt<DEG>(z, p);
+        f = permute_flags<DEG>(f, p);
+        store_pt<DEG>(W.z[nxt], d, z);
+        W.flg[nxt][d] = f;
+        W.theta[nxt][d] = th;
+    }
+}
+
+// ---- closed rows, segments and stitching (extended_source.py:156-727) for the final
+// points in buffer `buf`; writes the source's ExtContours
+template <int NL>
+__global__ __launch_bounds__(kExtBlock) void k_ext_contours(ExtArgs A, int buf) {
+    constexpr int DEG = ext_degree<NL>();
+    constexpr int NIMG = NL * NL + 1;
+    constexpr int NSEG = 3 * NIMG;  // nr_of_segments
+    __shared__ ActSeg s_con[3];
+    __shared__ unsigned s_allmask;
+    const int tid = threadIdx.x;
+    const int64_t src = blockIdx.x;
+    const ExtBatch B = ext_batch(A);
+    const int M = ext_points(B.npts[src], 10);
+    ExtWS W = ext_ws(ext_source_base<DEG>(A, src), DEG, A.mmax);
+    ExtContours* C = ext_contours<DEG>(A, src);
+    const cd* zb = W.z[buf];
+    const uint32_t* fb = W.flg[buf];
+    // closure test (extended_source.py:289-291)
+    if (tid == 0) s_allmask = 0xFFFFu;
+    __syncthreads();
+    unsigned am = 0xFFFFu;
+    for (int k = tid; k < M; k += kExtBlock) am &= fb[k];
+    atomicAnd(&s_allmask, am);
+    __syncthreads();
+    unsigned closed = 0;
+#pragma unroll
+    for (int r = 0; r < DEG; ++r) {
+        if ((s_allmask >> r) & 1u) {
+            if (norm2(pt<DEG>(zb, 0, r) - pt<DEG>(zb, M - 1, r)) < 1e-10) closed |= 1u << r;
         }
-        for (int r = 0; r < DEG; ++r) {
-            if ((closed >> r) & 1u) {
-                Piece pc{r, 0, M, 0};
-                const int par = ((fb[0] >> (16 + r)) & 1u) ? -1 : 1;
-                do_contour(&pc, 1, M, M, (double)(M + 1), par);
-            }
-        }
-        if (closed != (1u << DEG) - 1u) {
+    }
+    if (closed != (1u << DEG) - 1u) {
             // changepoints per boundary c in [0, M] (extended_source.py:163-188); rows
             // that are closed contribute nothing (their segments were zeroed)
             const unsigned open = ((1u << DEG) - 1u) & ~closed;
@@ -1258,16 +1422,91 @@ __global__ __launch_bounds__(kExtBlock) void k_mag_extended(ExtArgs A) {
                 }
             }
             __syncthreads();
+    }
+    if (tid == 0) {
+        C->closed = closed;
+        C->open = closed != (1u << DEG) - 1u;
+        if (C->open) {
+            for (int ci = 0; ci < 3; ++ci) C->con[ci] = s_con[ci];
+        }
+    }
+}
+
+// ---- contour integrals, magnification and its reverse mode (extended_source.py:
+// 860-901; integrate.py:18-121)
+template <int NL, bool LD, bool VJP>
+__global__ __launch_bounds__(kExtBlock) void k_ext_integrate(ExtArgs A, int buf) {
+    constexpr int DEG = ext_degree<NL>();
+    constexpr int ND = ext_nd<NL>();
+    __shared__ double s_d[8];
+    __shared__ double s_gl[4 * 128];
+    const int tid = threadIdx.x;
+    const int64_t src = blockIdx.x;
+    const ExtBatch B = ext_batch(A);
+    const int M = ext_points(B.npts[src], 10);
+    LensFrame<NL> lf;
+    cd w0;
+    double rho;
+    ext_source<NL>(A, src, lf, w0, rho);
+    const double u1 = LD ? A.u1[src * A.u1stride] : 0.0;
+    ExtWS W = ext_ws(ext_source_base<DEG>(A, src), DEG, A.mmax);
+    GLNodes g{};
+    if constexpr (LD) {
+        g.n1 = A.npts_ld / 2;
+        g.n2 = A.npts_ld - g.n1;
+        g.x1 = s_gl;
+        g.w1 = s_gl + 128;
+        g.x2 = s_gl + 256;
+        g.w2 = s_gl + 384;
+        gauss_legendre(g.n1, s_gl, s_gl + 128);
+        gauss_legendre(g.n2, s_gl + 256, s_gl + 384);
+    }
+    __syncthreads();
+    const cd* zb = W.z[buf];
+    const uint32_t* fb = W.flg[buf];
+    const double* thb = W.theta[buf];
+    double total = 0.0;
+    double gacc[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) gacc[d] = 0.0;
+    ExtTangents<NL> T;
+    if constexpr (VJP) ext_tangents<NL>(A.params + src * A.pstride, T);
+    auto do_contour = [&](const Piece* pcs, int npc, int L, int nterms, double cdiv, int par) {
+        const double I = integrate_contour<NL, DEG, LD>(zb, pcs, npc, L, nterms, cdiv, W.pq, lf,
+                                                        w0, rho, u1, g, s_d);
+        total += par * I;
+        if constexpr (VJP)
+            contour_adjoint<NL, DEG, LD, ND>(zb, thb, pcs, npc, L, nterms, cdiv, W.pq, lf, w0, rho,
+                                             u1, g, T, (double)par, gacc, s_d);
+    };
+    if constexpr (NL == 1) {
+        // both images, tail index N-1 (extended_source.py:860-866)
+        for (int r = 0; r < 2; ++r) {
+            Piece pc{r, 0, M, 0};
+            const int par = ((fb[0] >> (16 + r)) & 1u) ? -1 : 1;
+            do_contour(&pc, 1, M, M - 1, (double)M, par);
+        }
+    } else {
+        const ExtContours* C = ext_contours<DEG>(A, src);
+        const unsigned closed = C->closed;
+        for (int r = 0; r < DEG; ++r) {
+            if ((closed >> r) & 1u) {
+                Piece pc{r, 0, M, 0};
+                const int par = ((fb[0] >> (16 + r)) & 1u) ? -1 : 1;
+                do_contour(&pc, 1, M, M, (double)(M + 1), par);
+            }
+        }
+        if (C->open) {
             for (int ci = 0; ci < 3; ++ci) {
-                const ActSeg& a = s_con[ci];
+                const ActSeg& a = C->con[ci];
                 const int L = a.empty ? 0 : a.t + 1;
                 do_contour(a.pc, a.np, L, L, (double)(L + 1), a.par);
             }
         }
     }
-    const double m = fabs(total) / (PI * rho * rho);
+    const double m = fabs(total) / (kExtPi * rho * rho);
     if constexpr (VJP) {
-        const double base = (total >= 0.0 ? 1.0 : -1.0) / (PI * rho * rho);
+        const double base = (total >= 0.0 ? 1.0 : -1.0) / (kExtPi * rho * rho);
         const double gm = A.g_mag[src];
 #pragma unroll
         for (int d = 0; d < ND; ++d) {
@@ -1278,20 +1517,47 @@ __global__ __launch_bounds__(kExtBlock) void k_mag_extended(ExtArgs A) {
     }
     if (tid == 0) {
         A.mag[src] = m;
-        if (A.status) A.status[src] = s_fail ? 0 : 1;
+        if (A.status) A.status[src] = B.fail[src] ? 0 : 1;
     }
+}
+
+int ext_flat_grid(int64_t max_lanes);
+
+// the launch sequence for one batch of nlenses = NL
+template <int NL, int MODE, bool LD, bool VJP>
+int ext_run(hipStream_t s, const ExtArgs& a) {
+    const int64_t S = a.S;
+    const unsigned nb = (unsigned)S;
+    hipLaunchKernelGGL(k_ext_plan, dim3(1), dim3(kExtPlanBlock), 0, s, a);
+    const int64_t lanes_init = S * (int64_t)((ext_npts_init(a.npts_def) + kExtPer - 1) / kExtPer);
+    hipLaunchKernelGGL((k_ext_limb_init<NL, MODE>), dim3(ext_flat_grid(lanes_init)),
+                       dim3(kExtFlatBlock), 0, s, a);
+    hipLaunchKernelGGL((k_ext_chain<NL>), dim3(nb), dim3(kExtBlock), 0, s, a, 0, 0);
+    int P2 = 1;
+    while (P2 < a.mmax) P2 <<= 1;
+    const size_t lds = (size_t)P2 * (sizeof(uint64_t) + sizeof(int));
+    const int add_grid = ext_flat_grid(S * (int64_t)(a.nadd_max > 0 ? a.nadd_max : 1));
+    int cur = 0;
+    for (int round = 0; round < 10; ++round) {
+        hipLaunchKernelGGL((k_ext_select<NL>), dim3(nb), dim3(kExtBlock), lds, s, a, cur, round);
+        hipLaunchKernelGGL((k_ext_solve_add<NL, MODE>), dim3(add_grid), dim3(kExtFlatBlock), 0, s,
+                           a, cur);
+        cur ^= 1;
+    }
+    hipLaunchKernelGGL((k_ext_chain<NL>), dim3(nb), dim3(kExtBlock), 0, s, a, cur, 10);
+    if constexpr (NL > 1)
+        hipLaunchKernelGGL((k_ext_contours<NL>), dim3(nb), dim3(kExtBlock), 0, s, a, cur);
+    hipLaunchKernelGGL((k_ext_integrate<NL, LD, VJP>), dim3(nb), dim3(kExtBlock), 0, s, a, cur);
+    return hipGetLastError() == hipSuccess ? CAUS_OK : CAUS_ERR_LAUNCH;
 }
 
 template <int NL>
 int mag_extended_nl(hipStream_t s, const ExtArgs& a, int mode, bool ld, bool vjp);
 
-#define CAUS_EXT_CASE(NL, MODE, LD, VJP)                                                       \
-    hipLaunchKernelGGL((k_mag_extended<NL, MODE, LD, VJP>), grid, block, 0, s, a);              \
-    break;
+#define CAUS_EXT_CASE(NL, MODE, LD, VJP) return ext_run<NL, MODE, LD, VJP>(s, a);
 #define CAUS_DEFINE_EXT(NL)                                                                     \
     template <>                                                                                 \
     int mag_extended_nl<NL>(hipStream_t s, const ExtArgs& a, int mode, bool ld, bool vjp) {     \
-        const dim3 grid((unsigned)a.S), block(kExtBlock);                                       \
         switch ((mode == MODE_CF32 ? 4 : 0) + (ld ? 2 : 0) + (vjp ? 1 : 0)) {                   \
             case 0: CAUS_EXT_CASE(NL, MODE_FP64, false, false)                                  \
             case 1: CAUS_EXT_CASE(NL, MODE_FP64, false, true)                                   \
@@ -1304,7 +1570,6 @@ int mag_extended_nl(hipStream_t s, const ExtArgs& a, int mode, bool ld, bool vjp
             default:                                                                            \
                 return CAUS_ERR_ARG;                                                            \
         }                                                                                       \
-        return hipGetLastError() == hipSuccess ? CAUS_OK : CAUS_ERR_LAUNCH;                     \
     }
 
 }  // namespace caus
