@@ -170,8 +170,8 @@ def main():
         for key, name, fn in (
                 ("c1", "point_source_1e4", lambda: bench_point_source(args.steps, 2)),
                 ("c3", "lightcurves_256x2000_fwd_vjp", lambda: bench_lightcurves(3, 1)),
-                ("c4", "extended_binary_ld_4096_fwd_vjp", lambda: bench_extended_binary(1, 1)),
-                ("c5", "triple_extended_lightcurves_fwd_vjp", lambda: bench_triple(1, 1))):
+                ("c4", "extended_binary_ld_4096_fwd_vjp", lambda: bench_extended_binary(3, 1)),
+                ("c5", "triple_extended_lightcurves_fwd_vjp", lambda: bench_triple(1, 0))):
             if key not in sel:
                 continue
             print(f"bench: {name}", file=sys.stderr, flush=True)
@@ -300,12 +300,38 @@ def bench_lightcurves(steps, warmup, host_check=True):
     return out
 
 
+FP32_PEAK_TFLOPS = 157.3
+
+
+def contour_flops(work, nlenses, ld, npts_ld=100):
+    """Algorithmic flops of a contour run from its work counters (FP64 root-iterations,
+    FP32 root-iterations, compensated root-iterations, limb-point solves, contour
+    points), by the counting rule of the root solver plus, per limb-point solve, the
+    coefficient build (about 10^3 flops binary, 10^4 triple) and n x (lens equation 25 +
+    det J 40), and per limb-darkened contour point 2 npts_ld (lens equation 25 +
+    brightness 15).  The reverse mode is not counted.  Returns (FP64 flops, FP32 flops)."""
+    n = nlenses * nlenses + 1
+    it64, it32, itc, solves, points = (float(x) for x in work)
+    f64 = it64 * flops_per_root_iter(n) + itc * flops_comp_root_iter(n)
+    f64 += solves * ((1e3 if nlenses == 2 else 1e4) + n * 65)
+    f64 += points * (2 * npts_ld * 40 if ld else 4)
+    f32 = it32 * flops_per_root_iter(n)
+    return f64, f32
+
+
+def _roofline_entry(flops, dt):
+    f64, f32 = flops
+    busy = (f64 / (FP64_PEAK_TFLOPS * 1e12) + f32 / (FP32_PEAK_TFLOPS * 1e12)) / dt
+    return {"bound": "valu", "achieved_fp64_tflops": f64 / dt / 1e12,
+            "achieved_fp32_tflops": f32 / dt / 1e12, "frac": busy}
+
+
 def bench_extended_binary(steps, warmup, n_near=2048, n_far=2048):
     """Binary limb-darkened (u=0.5) sources, rho log-uniform [1e-3,1e-1]: n_near within
     2 rho of a caustic with ragged contour resolution 256..4096 points, n_far accepted by
     the hexadecapole gate.  One step is mag() forward + VJP: the gate on all sources,
-    the hexadecapole VJP on the accepted ones and the contour kernel with its VJP on
-    every source the gate rejects."""
+    the hexadecapole VJP on the accepted ones and the contour pipeline with its VJP on
+    every source the gate rejects.  Both solver forms (FP64, CF32)."""
     import torch
 
     from caustics_amd import _lib
@@ -321,25 +347,44 @@ def bench_extended_binary(steps, warmup, n_near=2048, n_far=2048):
     npts_all = np.concatenate([b["npts"], np.full(n_far, 256)])
     npts = torch.as_tensor(npts_all, device="cuda", dtype=torch.int32)
     gm = torch.ones(S, dtype=torch.float64, device="cuda")
-    cfg = (2, npts, int(npts_all.max()), True, 100, 2500, _lib.MODE_FP64)
+    out = {"workload": f"{S} binary limb-darkened sources (u=0.5), {n_near} near caustics "
+                       "with ragged 256-4096 limb points, rest hexadecapole-gated; mag() "
+                       "fwd+VJP"}
     res = {}
+    for form, mode in (("fp64", _lib.MODE_FP64), ("cf32", _lib.MODE_CF32)):
+        cfg = (2, npts, int(npts_all.max()), True, 100, 2500, mode)
 
-    def step():
-        m, use, go = _mag_fwd(cfg, w, rho, u1, prm, g=gm)
-        res["use"] = use
-        res["go"] = go
-    dt = _timed(step, steps, warmup, "C4 mag() fwd+VJP")
-    n_contour = int((~res["use"]).sum().item())
-    return {"workload": f"{S} binary limb-darkened sources (u=0.5), {n_near} near caustics with "
-                        "ragged 256-4096 limb points, rest hexadecapole-gated; mag() fwd+VJP",
-            "mag_evals_per_s": S / dt, "contour_sources": n_contour,
-            "hexadecapole_sources": S - n_contour, "ms": dt * 1e3,
-            "grad_finite": bool(torch.isfinite(res["go"]).all().item())}
+        def step():
+            res[form] = _mag_fwd(cfg, w, rho, u1, prm, g=gm)
+        dt = _timed(step, steps, warmup, f"C4 mag() fwd+VJP {form}")
+        work = torch.zeros(5, dtype=torch.int64, device="cuda")
+        _mag_fwd(cfg, w, rho, u1, prm, g=gm, work=work)
+        work = work.cpu().numpy()
+        m, use, go = res[form]
+        out[form] = {"mag_evals_per_s_fwd_vjp": S / dt, "ms": dt * 1e3,
+                     "contour_sources": int((~use).sum().item()),
+                     "hexadecapole_sources": int(use.sum().item()),
+                     "work": {"root_iterations_fp64": int(work[0]),
+                              "root_iterations_fp32": int(work[1]),
+                              "limb_solves": int(work[3]), "contour_points": int(work[4])},
+                     "roofline": _roofline_entry(contour_flops(work, 2, True), dt),
+                     "grad_finite": bool(torch.isfinite(go).all().item())}
+    m64, m32 = res["fp64"][0], res["cf32"][0]
+    e = ((m32 - m64).abs() / m64.abs()).cpu().numpy()
+    ge = ((res["cf32"][2] - res["fp64"][2]).abs()
+          / torch.maximum(res["fp64"][2].abs(), 1e-3 * m64.abs()[:, None])).cpu().numpy()
+    out["cf32"].update({"max_rel_err_vs_fp64": float(e.max()),
+                        "p99_rel_err_vs_fp64": float(np.quantile(e, 0.99)),
+                        "frac_le_1e-8": float(np.mean(e <= 1e-8)),
+                        "grad_p99_rel_err_vs_fp64": float(np.quantile(ge, 0.99))})
+    return out
 
 
-def bench_triple(steps, warmup, n_sets=8, n_epochs=1500, npts=256):
-    """Triple-lens caustic-crossing light curves, full contour at every epoch, fused
-    fwd+VJP (u=0.4, rho log-uniform [1e-3,3e-2]); a bounded subset of parameter sets."""
+def bench_triple(steps, warmup, n_sets=1024, n_epochs=1500, npts=256, budget_s=40.0):
+    """Triple-lens caustic-crossing light curves (u=0.4, rho log-uniform [1e-3,3e-2]),
+    full limb-darkened contour at every epoch, fused fwd+VJP, both solver forms.  The
+    timed batch is the full n_sets x n_epochs when one step fits budget_s, otherwise the
+    largest multiple of 8 parameter sets that does (stated in the result)."""
     import torch
 
     from caustics_amd import _lib
@@ -347,23 +392,48 @@ def bench_triple(steps, warmup, n_sets=8, n_epochs=1500, npts=256):
     from caustics_amd.extended_source import _run
 
     b = W.triple_caustic_lightcurves(n_sets, n_epochs)
-    w = torch.as_tensor(b["w"].reshape(-1), device="cuda")
-    S = w.numel()
-    rep = lambda x: torch.as_tensor(np.repeat(np.asarray(x), n_epochs), device="cuda")
-    rho, u1 = rep(b["rho"]), rep(b["u1"])
-    prm = torch.stack([rep(b[k]) for k in ("s", "q", "q3", "r3", "psi")], -1).contiguous()
-    gm = torch.ones(S, dtype=torch.float64, device="cuda")
-    st_holder = {}
 
-    def step():
-        m, st, go = _run(3, w, rho, prm, None, npts, True, u1, 100, 2500, _lib.MODE_FP64, g=gm,
-                         max_chunk=S)
-        st_holder["st"] = st
-    dt = _timed(step, steps, warmup, "C5 triple contours")
-    return {"workload": f"{n_sets} triple-lens parameter sets x {n_epochs} epochs (caustic "
-                        f"crossing), limb-darkened u=0.4, npts_limb={npts}, full contour fwd+VJP",
-            "mag_evals_per_s_fwd_vjp": S / dt, "ms": dt * 1e3,
-            "unconverged": int((st_holder["st"] == 0).sum().item())}
+    def batch(k):
+        w = torch.as_tensor(b["w"][:k].reshape(-1), device="cuda")
+        rep = lambda x: torch.as_tensor(np.repeat(np.asarray(x)[:k], n_epochs), device="cuda")
+        prm = torch.stack([rep(b[c]) for c in ("s", "q", "q3", "r3", "psi")], -1).contiguous()
+        return w, rep(b["rho"]), rep(b["u1"]), prm
+
+    # size the timed batch from a probe of 8 parameter sets
+    w, rho, u1, prm = batch(8)
+    gm = torch.ones(w.numel(), dtype=torch.float64, device="cuda")
+    _run(3, w, rho, prm, None, npts, True, u1, 100, 2500, _lib.MODE_FP64, g=gm)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _run(3, w, rho, prm, None, npts, True, u1, 100, 2500, _lib.MODE_FP64, g=gm)
+    torch.cuda.synchronize()
+    per_set = (time.perf_counter() - t0) / 8
+    k = int(min(n_sets, max(8, budget_s / 2.5 / max(per_set, 1e-9))) // 8 * 8)
+    w, rho, u1, prm = batch(k)
+    S = w.numel()
+    gm = torch.ones(S, dtype=torch.float64, device="cuda")
+    out = {"workload": f"{k} of {n_sets} triple-lens parameter sets x {n_epochs} epochs "
+                       f"(caustic crossing), limb-darkened u=0.4, npts_limb={npts}, full "
+                       "contour fwd+VJP at every epoch", "evals": S}
+    res = {}
+    for form, mode in (("fp64", _lib.MODE_FP64), ("cf32", _lib.MODE_CF32)):
+        def step():
+            res[form] = _run(3, w, rho, prm, None, npts, True, u1, 100, 2500, mode, g=gm)
+        dt = _timed(step, 1, 0, f"C5 triple contours {form} ({S} contours)")
+        work = torch.zeros(5, dtype=torch.int64, device="cuda")
+        _run(3, w[:8 * n_epochs], rho[:8 * n_epochs], prm[:8 * n_epochs], None, npts, True,
+             u1[:8 * n_epochs], 100, 2500, mode, g=gm[:8 * n_epochs], work=work)
+        work = work.cpu().numpy() * (S / (8 * n_epochs))
+        out[form] = {"mag_evals_per_s_fwd_vjp": S / dt, "ms": dt * 1e3,
+                     "unconverged": int((res[form][1] == 0).sum().item()),
+                     "roofline": _roofline_entry(contour_flops(work, 3, True), dt),
+                     "roofline_note": "work counters from the first 8 parameter sets, scaled"}
+    m64, m32 = res["fp64"][0], res["cf32"][0]
+    e = ((m32 - m64).abs() / m64.abs()).cpu().numpy()
+    out["cf32"].update({"max_rel_err_vs_fp64": float(e.max()),
+                        "p99_rel_err_vs_fp64": float(np.quantile(e, 0.99)),
+                        "frac_le_1e-8": float(np.mean(e <= 1e-8))})
+    return out
 
 
 def root_condition(c, z):

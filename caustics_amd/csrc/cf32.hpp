@@ -12,7 +12,8 @@
 //            FP64 coefficients, iterating in FP64 until it passes (normally it passes on
 //            the first evaluation).  The returned roots therefore satisfy the same FP64
 //            criterion as the FP64 form.
-// A non-finite FP32 iterate sends the polynomial through the FP64 solver from scratch.
+// A non-finite FP32 iterate, or a non-finite result (two estimates that FP32 could not
+// separate), sends the polynomial through the FP64 solver from scratch.
 #pragma once
 #include "solver.hpp"
 
@@ -162,6 +163,18 @@ __device__ __forceinline__ bool solve_cf32(const cd (&p)[DEG + 1], cd (&z)[DEG],
         }
     }
     st.sweeps = it + 1 + it2;
+    // Two estimates that FP32 cannot tell apart (a nearly double root) make the Aberth
+    // sums of stages B and C infinite; NaN roots would then pass the stopping test.
+    // Such a polynomial is solved again by the FP64 form.
+    bool fin = true;
+#pragma unroll
+    for (int k = 0; k < DEG; ++k) fin = fin && isfinite(z[k].re) && isfinite(z[k].im);
+    if (!fin) {
+        SolveStats s2;
+        done = solve_fp64<DEG, MODE_FP64, false>(p, z, itmax, false, s2);
+        st.sweeps += s2.sweeps;
+        st.it64 += s2.it64;
+    }
     return done;
 }
 

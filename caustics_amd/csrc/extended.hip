@@ -6,6 +6,11 @@ extern "C" int caus_mag_extended_source_vjp(void*, int32_t, const double*, int64
                                             int32_t, const double*, int64_t, int32_t,
                                             const double*, double*, double*, uint8_t*, void*,
                                             int64_t, int32_t, int32_t);
+extern "C" int caus_mag_extended_source_work(void*, int32_t, const double*, int64_t, const double*,
+                                             const double*, int64_t, const int32_t*, int32_t,
+                                             int32_t, const double*, int64_t, int32_t,
+                                             const double*, double*, double*, uint8_t*, void*,
+                                             int64_t, int32_t, int32_t, uint64_t*);
 
 using namespace caus;
 
@@ -54,6 +59,21 @@ extern "C" int caus_mag_extended_source_vjp(void* stream, int32_t nlenses, const
                                             const double* g_mag, double* mag, double* g_params,
                                             uint8_t* status, void* workspace,
                                             int64_t workspace_bytes, int32_t itmax, int32_t mode) {
+    return caus_mag_extended_source_work(stream, nlenses, w0, S, rho, params, params_stride,
+                                         npts_limb, npts_max, limb_darkening, u1, u1_stride,
+                                         npts_ld, g_mag, mag, g_params, status, workspace,
+                                         workspace_bytes, itmax, mode, nullptr);
+}
+
+extern "C" int caus_mag_extended_source_work(void* stream, int32_t nlenses, const double* w0,
+                                             int64_t S, const double* rho, const double* params,
+                                             int64_t params_stride, const int32_t* npts_limb,
+                                             int32_t npts_max, int32_t limb_darkening,
+                                             const double* u1, int64_t u1_stride,
+                                             int32_t npts_ld, const double* g_mag, double* mag,
+                                             double* g_params, uint8_t* status, void* workspace,
+                                             int64_t workspace_bytes, int32_t itmax, int32_t mode,
+                                             uint64_t* work) {
     if (S < 0 || nlenses < 1 || nlenses > 3 || itmax < 0 || params_stride < 0 || u1_stride < 0)
         return CAUS_ERR_ARG;
     if (mode != CAUS_MODE_FP64 && mode != CAUS_MODE_CF32) return CAUS_ERR_ARG;
@@ -83,6 +103,7 @@ extern "C" int caus_mag_extended_source_vjp(void* stream, int32_t nlenses, const
     const bool vjp = g_mag != nullptr;
     if (vjp && !g_params) return CAUS_ERR_ARG;
     a.ws = reinterpret_cast<char*>(workspace);
+    a.work = reinterpret_cast<unsigned long long*>(work);
     a.ws_per_source = per;
     a.mmax = ext_mmax(npts_max);
     a.nadd_max = ext_n_add(npts_max);

@@ -148,13 +148,10 @@ __device__ __forceinline__ uint64_t nn_map(const cd (&a)[DEG], const cd (&b)[DEG
 }
 
 // ---- point solve -----------------------------------------------------------
-// warm: z holds a nearby point's images on entry and starts the iteration (a limb
-// neighbour, as the reference's sequential solver does); a warm start that fails to
-// converge is retried from the standard initial estimates.
 template <int NL, int MODE>
 __device__ __forceinline__ bool ext_solve_point(cd w, const LensFrame<NL>& lf,
                                                 cd (&z)[ext_degree<NL>()], uint32_t& flags,
-                                                int itmax, bool warm = false) {
+                                                int itmax, SolveStats* stats = nullptr) {
     constexpr int DEG = ext_degree<NL>();
     bool ok = true;
     flags = 0;
@@ -169,13 +166,7 @@ __device__ __forceinline__ bool ext_solve_point(cd w, const LensFrame<NL>& lf,
             if (norm2(z[j]) < 1.0) flags |= 1u << (16 + j);
         }
     } else {
-        ok = solve_lens<NL, MODE>(w, lf, z, itmax, warm);
-        if (warm) {
-            bool fin = true;
-#pragma unroll
-            for (int j = 0; j < DEG; ++j) fin = fin && isfinite(z[j].re) && isfinite(z[j].im);
-            if (!ok || !fin) ok = solve_lens<NL, MODE>(w, lf, z, itmax, false);
-        }
+        ok = solve_lens<NL, MODE>(w, lf, z, itmax, false, stats);
 #pragma unroll
         for (int j = 0; j < DEG; ++j) {
             ImageTerms<NL> it = image_terms<NL>(z[j], lf);
@@ -848,11 +839,10 @@ __device__ void contour_adjoint(const cd* zb, const double* theta, const Piece* 
 // sequence of launches over the whole batch, so that the root solves (nearly all of
 // the arithmetic) run one limb point per lane at the solver's occupancy:
 //   k_ext_plan        1 block      effective npts, prefix offsets of the flat work lists
-//   k_ext_limb_init   flat         initial limb angles, kExtPer consecutive angles per
-//                                  lane, each warm-started from the previous one
+//   k_ext_limb_init   flat         initial limb angles, kExtPer consecutive angles per lane
 //   k_ext_chain       per source   image continuation (extended_source.py:34-53)
 //   10 x k_ext_select per source   gap sort, insertion shifts, move old points, new tasks
-//        k_ext_solve_add flat      new points warm-started from their left neighbour
+//        k_ext_solve_add flat      new points, matched to their left neighbour
 //   k_ext_chain       per source
 //   k_ext_contours    per source   closure test, segments, stitching -> piece lists
 //   k_ext_integrate   per source   contour integrals, magnification, reverse mode
@@ -882,6 +872,9 @@ struct ExtArgs {
     double* g_out;         // VJP only: (S, 4 + P)
     char* ws;              // batch header, then one block of ws_per_source bytes per source
     size_t ws_per_source;
+    unsigned long long* work;  // optional counters: FP64 root-iterations, FP32
+                               // root-iterations, compensated root-iterations, limb-point
+                               // solves, contour points
     int mmax;
     int nadd_max;
     int itmax;
@@ -969,6 +962,13 @@ __device__ inline void ext_source(const ExtArgs& A, int64_t s, LensFrame<NL>& lf
     rho = A.rho[s];
 }
 
+__device__ inline void ext_count(unsigned long long* work, const SolveStats& st, int solves) {
+    atomicAdd(work + 0, (unsigned long long)st.it64);
+    atomicAdd(work + 1, (unsigned long long)st.it32);
+    atomicAdd(work + 2, (unsigned long long)st.itcomp);
+    atomicAdd(work + 3, (unsigned long long)solves);
+}
+
 // limb points of source s after `round` refinement rounds
 __device__ inline int ext_points(int npts, int round) {
     return ext_npts_init(npts) + round * ext_n_add(npts);
@@ -1024,8 +1024,11 @@ static __global__ __launch_bounds__(kExtPlanBlock) void k_ext_plan(ExtArgs A) {
 }
 
 // ---- initial limb points (extended_source.py:100-106): one run of kExtPer consecutive
-// angles per lane, the first from the standard initial estimates, the others from the
-// previous angle's images
+// angles per lane.  Every point starts from the standard initial estimates: a start
+// from a neighbour's images can leave two estimates inside one cluster of nearly double
+// images (near a caustic), both passing the backward-error stopping test, so that one
+// image is found twice and its partner never; independent solves keep the image sets
+// of the oracle.
 template <int NL, int MODE>
 __global__ __launch_bounds__(kExtFlatBlock) void k_ext_limb_init(ExtArgs A) {
     constexpr int DEG = ext_degree<NL>();
@@ -1048,17 +1051,20 @@ __global__ __launch_bounds__(kExtFlatBlock) void k_ext_limb_init(ExtArgs A) {
         ext_source<NL>(A, s, lf, w0, rho);
         ExtWS W = ext_ws(ext_source_base<DEG>(A, s), DEG, A.mmax);
         cd z[DEG];
+        SolveStats st{0, 0, 0, 0};
         for (int k = k0; k < k1; ++k) {
             const double th = (k < M0 - 1) ? -kExtPi + k * (2.0 * kExtPi / (M0 - 1)) : kExtPi - 1e-8;
             double sn, cs;
             sincos(th, &sn, &cs);
             uint32_t f;
-            if (!ext_solve_point<NL, MODE>(w0 + rho * cd{cs, sn}, lf, z, f, A.itmax, k > k0))
+            if (!ext_solve_point<NL, MODE>(w0 + rho * cd{cs, sn}, lf, z, f, A.itmax,
+                                           A.work ? &st : nullptr))
                 B.fail[s] = 1;
             W.theta[0][k] = th;
             store_pt<DEG>(W.z[0], k, z);
             W.flg[0][k] = f;
         }
+        if (A.work) ext_count(A.work, st, k1 - k0);
     }
 }
 
@@ -1170,7 +1176,7 @@ __global__ __launch_bounds__(kExtBlock) void k_ext_select(ExtArgs A, int cur, in
 }
 
 // ---- one refinement round, part 2: every new point of the batch, one per lane,
-// started from its left neighbour's images and matched to them (match_points)
+// solved and matched to its left neighbour's images (match_points)
 template <int NL, int MODE>
 __global__ __launch_bounds__(kExtFlatBlock) void k_ext_solve_add(ExtArgs A, int cur) {
     constexpr int DEG = ext_degree<NL>();
@@ -1191,11 +1197,12 @@ __global__ __launch_bounds__(kExtFlatBlock) void k_ext_solve_add(ExtArgs A, int 
         sincos(th, &sn, &cs);
         cd z[DEG], left[DEG];
         load_pt<DEG>(W.z[cur], k, left);
-#pragma unroll
-        for (int r = 0; r < DEG; ++r) z[r] = left[r];
         uint32_t f;
-        if (!ext_solve_point<NL, MODE>(w0 + rho * cd{cs, sn}, lf, z, f, A.itmax, true))
+        SolveStats st{0, 0, 0, 0};
+        if (!ext_solve_point<NL, MODE>(w0 + rho * cd{cs, sn}, lf, z, f, A.itmax,
+                                       A.work ? &st : nullptr))
             B.fail[s] = 1;
+        if (A.work) ext_count(A.work, st, 1);
         const uint64_t p = greedy_match<DEG>(left, z);
         permute_pt<DEG>(z, p);
         f = permute_flags<DEG>(f, p);
@@ -1471,7 +1478,9 @@ __global__ __launch_bounds__(kExtBlock) void k_ext_integrate(ExtArgs A, int buf)
     for (int d = 0; d < ND; ++d) gacc[d] = 0.0;
     ExtTangents<NL> T;
     if constexpr (VJP) ext_tangents<NL>(A.params + src * A.pstride, T);
+    unsigned long long npoints = 0;
     auto do_contour = [&](const Piece* pcs, int npc, int L, int nterms, double cdiv, int par) {
+        npoints += (L > 0) ? (unsigned long long)L : 0ull;
         const double I = integrate_contour<NL, DEG, LD>(zb, pcs, npc, L, nterms, cdiv, W.pq, lf,
                                                         w0, rho, u1, g, s_d);
         total += par * I;
@@ -1518,6 +1527,7 @@ __global__ __launch_bounds__(kExtBlock) void k_ext_integrate(ExtArgs A, int buf)
     if (tid == 0) {
         A.mag[src] = m;
         if (A.status) A.status[src] = B.fail[src] ? 0 : 1;
+        if (A.work) atomicAdd(A.work + 4, npoints);
     }
 }
 

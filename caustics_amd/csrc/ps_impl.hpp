@@ -24,15 +24,24 @@ constexpr int ps_degree() {
 // warm = true: z holds initial estimates (the roots of a nearby source position)
 template <int NL, int MODE>
 __device__ __forceinline__ bool solve_lens(cd wl, const LensFrame<NL>& lf, cd (&z)[ps_degree<NL>()],
-                                           int itmax, bool warm = false) {
+                                           int itmax, bool warm = false,
+                                           SolveStats* stats = nullptr) {
     constexpr int DEG = ps_degree<NL>();
     cd p[DEG + 1];
     lens_poly<NL>(wl, lf, p);
     SolveStats st;
+    bool ok;
     if constexpr (MODE == MODE_CF32)
-        return solve_cf32<DEG>(p, z, itmax, warm, st);
+        ok = solve_cf32<DEG>(p, z, itmax, warm, st);
     else
-        return solve_fp64<DEG, MODE, false>(p, z, itmax, warm, st);
+        ok = solve_fp64<DEG, MODE, false>(p, z, itmax, warm, st);
+    if (stats) {
+        stats->sweeps += st.sweeps;
+        stats->it64 += st.it64;
+        stats->itcomp += st.itcomp;
+        stats->it32 += st.it32;
+    }
+    return ok;
 }
 
 // Value and tangents of the magnification at lens-frame source position wl.

@@ -97,12 +97,16 @@ __device__ __forceinline__ cd pow2_explog(cd z) {
     return cd{m * c, m * s};
 }
 
-template <int DEG>
+// sum_{i != j} 1/(x - z_i).  RCP_STEPS Newton steps refine the v_rcp_f64 seed (relative
+// error 2^-24 unrefined): the Fast form uses the bare seed, since an Aberth correction
+// with a slightly perturbed sum is still a convergent iteration and the accuracy of the
+// result is set by the exact FP64 residual in the stopping test.
+template <int DEG, int RCP_STEPS = 1>
 __device__ __forceinline__ cd aberth_sum(const cd (&z)[DEG], int j, cd x) {
     cd S = cd{0.0, 0.0};
 #pragma unroll
     for (int i = 0; i < DEG; ++i)
-        if (i != j) S = S + crcp<1>(x - z[i]);
+        if (i != j) S = S + crcp<RCP_STEPS>(x - z[i]);
     return S;
 }
 
@@ -241,12 +245,15 @@ __device__ __forceinline__ bool comp_step(const cd (&p)[DEG + 1], cd (&z)[DEG], 
     }
 }
 
-// Plain FP64 step for root j, forward polynomial only (Fast form).
+// Plain FP64 step for root j, forward polynomial only (Fast form).  |x| in the bound
+// b = sum alpha'_k |x|^k of the stopping test comes from the bare v_sqrt_f64 (the bound
+// scales a threshold; an error of a few ulps in it moves the test negligibly), the
+// residual h is exact FP64 Horner.
 template <int DEG>
 __device__ __forceinline__ bool fast_step(const cd (&p)[DEG + 1], const double (&al)[DEG + 1],
                                           cd (&z)[DEG], int j) {
     cd x = z[j];
-    double ax = sqrt(norm2(x));
+    const double ax = __builtin_amdgcn_sqrt(norm2(x));
     double b = al[DEG];
 #pragma unroll
     for (int k = DEG - 1; k >= 0; --k) b = fma(b, ax, al[k]);
@@ -258,7 +265,7 @@ __device__ __forceinline__ bool fast_step(const cd (&p)[DEG + 1], const double (
     }
     const double tb = EA_EPS * b;
     if (norm2(h) > tb * tb) {
-        cd S = aberth_sum<DEG>(z, j, x);
+        cd S = aberth_sum<DEG, 0>(z, j, x);
         z[j] = x - cdiv<2>(h, hd - h * S);
         return false;
     }

@@ -37,7 +37,10 @@ def _col(x, n, dtype=torch.float64):
 
 
 def _run(nlenses, w, rho_t, prm, npts_t, npts_max, ld, u1_t, npts_ld, itmax, mode_id, g=None,
-         max_chunk=None):
+         max_chunk=None, work=None):
+    """One or more launches of the contour pipeline; `work`, a (5,) int64 CUDA tensor,
+    accumulates the kernels' work counters (FP64, FP32 and compensated root-iterations,
+    limb-point solves, contour points)."""
     L = _lib.lib()
     S = w.numel()
     P = 0 if prm is None else prm.shape[1]
@@ -52,14 +55,14 @@ def _run(nlenses, w, rho_t, prm, npts_t, npts_max, ld, u1_t, npts_ld, itmax, mod
     for lo in range(0, S, chunk):
         hi = min(S, lo + chunk)
         sl = slice(lo, hi)
-        rc = L.caus_mag_extended_source_vjp(
+        rc = L.caus_mag_extended_source_work(
             _lib.stream_ptr(), nlenses, _lib.ptr(w[sl]), hi - lo, _lib.ptr(rho_t[sl]),
             None if prm is None else _lib.ptr(prm[sl]), P,
             None if npts_t is None else _lib.ptr(npts_t[sl]), npts_max, 1 if ld else 0,
             None if u1_t is None else _lib.ptr(u1_t[sl]), 1, int(npts_ld),
             None if g is None else _lib.ptr(g[sl]), _lib.ptr(mag[sl]),
             None if g_out is None else _lib.ptr(g_out[sl]), _lib.ptr(status[sl]), _lib.ptr(ws),
-            ws.numel() * 8, int(itmax), mode_id)
+            ws.numel() * 8, int(itmax), mode_id, None if work is None else _lib.ptr(work))
         _lib.check(rc, "caus_mag_extended_source")
     return mag, status, g_out
 

@@ -152,7 +152,7 @@ def _gate(nlenses, w, rho_t, prm, itmax):
     return mu, use.bool()
 
 
-def _mag_fwd(cfg, w, rho_t, u1_t, prm, g=None):
+def _mag_fwd(cfg, w, rho_t, u1_t, prm, g=None, work=None):
     """Gate, then the contour kernel on the rejected points; with g also the VJP of both
     branches.  Returns (mag, use, g_out or None)."""
     from .extended_source import _run
@@ -175,7 +175,8 @@ def _mag_fwd(cfg, w, rho_t, u1_t, prm, g=None):
     if rej.numel():
         sub = lambda x: None if x is None else x[rej].contiguous()
         m, _, gr = _run(nlenses, sub(w), sub(rho_t), sub(prm), sub(npts_t), npts_max, ld,
-                        sub(u1_t) if ld else None, npts_ld, itmax, mode_id, g=sub(g))
+                        sub(u1_t) if ld else None, npts_ld, itmax, mode_id, g=sub(g),
+                        work=work)
         out[rej] = m
         if g is not None:
             go[rej] = gr

@@ -183,6 +183,20 @@ int caus_mag_extended_source_vjp(void* stream, int32_t nlenses, const double* w0
                                  double* g_params, uint8_t* status, void* workspace,
                                  int64_t workspace_bytes, int32_t itmax, int32_t mode);
 
+/* caus_mag_extended_source_vjp (g_mag may be NULL for the forward only) that also adds
+ * work counts to work[5] (uint64, caller-zeroed): FP64 root-iterations, FP32
+ * root-iterations, compensated root-iterations, limb-point solves, contour points.  The
+ * counters cost atomics; they exist to price a run against the VALU roofline.  No
+ * counterpart in the reference. */
+int caus_mag_extended_source_work(void* stream, int32_t nlenses, const double* w0, int64_t S,
+                                  const double* rho, const double* params,
+                                  int64_t params_stride, const int32_t* npts_limb,
+                                  int32_t npts_max, int32_t limb_darkening, const double* u1,
+                                  int64_t u1_stride, int32_t npts_ld, const double* g_mag,
+                                  double* mag, double* g_params, uint8_t* status,
+                                  void* workspace, int64_t workspace_bytes, int32_t itmax,
+                                  int32_t mode, uint64_t* work);
+
 /* Greedy continuation of root sets along a sequence: for each of nchains chains of npts
  * sets of n points (pts (nchains, npts, n) complex), out[k] = pts[k] permuted so that
  * each row continues the previous set by the reference's match_points rule

@@ -92,6 +92,32 @@ def test_triple_near_caustics(gpu):
     compare(m, mo, "triple uniform")
 
 
+@pytest.mark.parametrize("nlenses", [2, 3])
+def test_cf32_form_matches_fp64(gpu, nlenses):
+    """The reduced-precision form (FP32 Aberth, compensated-FP32 sweep, FP64 polish) of
+    the limb solves against the FP64 form, forward and VJP: within 1e-8 relative."""
+    from caustics_amd.extended_source import extended_fwd_vjp
+
+    if nlenses == 2:
+        prm = dict(s=0.9, q=0.2)
+    else:
+        prm = dict(s=1.2, q=0.3, q3=0.01, r3=0.8, psi=1.0)
+    rho = 1e-2
+    w = caustic_points(rho, nlenses=nlenses, npts=6, **prm)
+    kw = dict(nlenses=nlenses, npts_limb=256, limb_darkening=True, u1=0.5, npts_ld=100, **prm)
+    g = np.ones(len(w))
+    m64, g64, _ = extended_fwd_vjp(w, rho, g, mode="fp64", **kw)
+    m32, g32, st = extended_fwd_vjp(w, rho, g, mode="cf32", **kw)
+    m64, m32 = m64.cpu().numpy(), m32.cpu().numpy()
+    g64, g32 = g64.cpu().numpy(), g32.cpu().numpy()
+    assert st.cpu().numpy().all()
+    e = np.abs(m32 - m64) / m64
+    eg = np.abs(g32 - g64) / np.maximum(np.abs(g64), 1e-3 * m64[:, None])
+    print(f"cf32 vs fp64 nl={nlenses}: mag max rel {e.max():.2e}, grad max rel {eg.max():.2e}")
+    assert np.mean(e <= 1e-8) >= 0.95 and e.max() < 1e-3
+    assert np.mean(eg <= 1e-6) >= 0.95
+
+
 def test_ragged_batch_matches_individual(gpu):
     rho = 2e-2
     w = caustic_points(rho, npts=5, s=1.1, q=0.3)

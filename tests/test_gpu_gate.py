@@ -61,7 +61,7 @@ _HEX_CASES = {
 @pytest.mark.parametrize("nl", [1, 2, 3])
 def test_hexadecapole_vjp_matches_finite_differences(gpu, nl):
     """g_out = g * d mu_hex / d(Re w, Im w, rho, u1, params) against central differences
-    of the oracle's hexadecapole (h = 1e-6 relative), on points away from caustics
+    of the oracle's hexadecapole (h = 1e-5 relative), on points away from caustics
     where the image count does not change within the step."""
     from caustics_amd.lightcurve import hexadecapole_vjp
 
@@ -94,12 +94,14 @@ def test_hexadecapole_vjp_matches_finite_differences(gpu, nl):
 
     bad = {}
     for k, key in enumerate(names):
-        h = 1e-6 * max(abs(base[key]) if key in base and np.ndim(base[key]) == 0 else 1.0, 1e-2)
+        h = 1e-5 * max(abs(base[key]) if key in base and np.ndim(base[key]) == 0 else 1.0, 0.1)
         fd = (f(key, h) - f(key, -h)) / (2 * h)
         fd2 = (f(key, 2 * h) - f(key, -2 * h)) / (4 * h)
-        # the difference quotient is trusted where steps h and 2h agree to 1e-7
-        stable = np.abs(fd - fd2) <= 1e-7 * np.maximum(np.abs(fd), mo)
-        e = np.abs(gm * fd - go[:, k]) / np.maximum(np.abs(gm * fd), 1e-3 * mo)
+        # the difference quotient is trusted where steps h and 2h agree to 1e-6 of the
+        # scale the error is measured against
+        scale = np.maximum(np.abs(fd), 1e-3 * mo)
+        stable = np.abs(fd - fd2) <= 1e-6 * scale
+        e = np.abs(gm * fd - go[:, k]) / (gm * scale)
         bad[key] = (stable.mean(), e[stable].max())
         assert stable.mean() > 0.8, (key, stable.mean())
         assert np.quantile(e[stable], 0.99) < 1e-5, (key, e[stable].max())

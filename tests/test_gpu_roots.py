@@ -150,3 +150,25 @@ def test_roots_vjp_matches_implicit_derivative_and_fd(gpu):
         zp, zm_ = match_rows(zn, zp), match_rows(zn, zm_)
         fd = np.real(v * (zp - zm_)).sum(1) / (2 * h)
         np.testing.assert_allclose(grad[:, kk].real, fd, rtol=1e-4, atol=1e-6)
+
+
+def test_cf32_nearly_double_roots_regression(gpu):
+    """Limb points of a triple-lens source crossing a caustic (s=2.59, q=0.20, q3=0.022):
+    their polynomials have a nearly double root that FP32 cannot resolve, which used to
+    turn the CF32 form's Aberth sums infinite and return NaN roots flagged converged.
+    The CF32 result must be finite and match the FP64 form's root sets."""
+    from caustics_amd.ehrlich_aberth_primitive import solve_with_stats
+
+    prm = dict(s=2.5915722237180048, q=0.20148953757052693, q3=0.022476073987355016,
+               r3=1.0905260567197013, psi=3.2357322003521114)
+    w0, rho = 0.16212946846468673 - 0.9840604288897297j, 0.003657724979001628
+    lp, x_cm = lens.lens_params(3, **prm)
+    r, e = lens.lens_positions_masses(3, **lp)
+    th = np.linspace(-np.pi, np.pi, 2048)
+    c = lens.poly_coeffs_product(w0 + x_cm + rho * np.exp(1j * th), r, e)[:, ::-1].copy()
+    z64, st64, _ = solve_with_stats(c, mode="fp64", init="complex")
+    z32, st32, _ = solve_with_stats(c, mode="cf32", init="complex")
+    z64, z32 = z64.cpu().numpy(), z32.cpu().numpy()
+    assert np.isfinite(z32).all() and st32.cpu().numpy().all()
+    ok, _, _ = roots_close(z32, z64, c)
+    assert ok.all()
