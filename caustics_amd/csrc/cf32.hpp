@@ -105,14 +105,9 @@ __device__ __forceinline__ bool solve_cf32(const cd (&p)[DEG + 1], cd (&z)[DEG],
     bool finite = true;
 #pragma unroll
     for (int k = 0; k < DEG; ++k) finite = finite && isfinite(zf[k].re) && isfinite(zf[k].im);
-    if (!finite) {
-        SolveStats s2;
-        bool ok = solve_fp64<DEG, MODE_FP64, false>(p, z, itmax, false, s2);
-        st.sweeps = it + s2.sweeps;
-        st.it64 = s2.it64;
-        return ok;
-    }
-
+    bool done = false;
+    int it2 = 0;
+    if (finite) {
     // ---- stage B: one compensated FP32 sweep, result in double-float
 #pragma unroll
     for (int j = 0; j < DEG; ++j) {
@@ -146,8 +141,6 @@ __device__ __forceinline__ bool solve_cf32(const cd (&p)[DEG + 1], cd (&z)[DEG],
 
     // ---- stage C: FP64 polish under the FP64 stopping test
     cx = 0;
-    bool done = false;
-    int it2 = 0;
     for (; it2 < itmax; ++it2) {
 #pragma unroll
         for (int j = 0; j < DEG; ++j) {
@@ -163,16 +156,16 @@ __device__ __forceinline__ bool solve_cf32(const cd (&p)[DEG + 1], cd (&z)[DEG],
         }
     }
     st.sweeps = it + 1 + it2;
-    // Two estimates that FP32 cannot tell apart (a nearly double root) make the Aberth
-    // sums of stages B and C infinite; NaN roots would then pass the stopping test.
-    // Such a polynomial is solved again by the FP64 form.
-    bool fin = true;
 #pragma unroll
-    for (int k = 0; k < DEG; ++k) fin = fin && isfinite(z[k].re) && isfinite(z[k].im);
-    if (!fin) {
+    for (int k = 0; k < DEG; ++k) finite = finite && isfinite(z[k].re) && isfinite(z[k].im);
+    }  // finite after stage A
+    // A non-finite FP32 iterate, or two estimates that FP32 cannot tell apart (a nearly
+    // double root: the Aberth sums of stages B and C turn infinite and NaN roots would
+    // pass the stopping test): the polynomial is solved again by the FP64 form.
+    if (!finite) {
         SolveStats s2;
         done = solve_fp64<DEG, MODE_FP64, false>(p, z, itmax, false, s2);
-        st.sweeps += s2.sweeps;
+        st.sweeps = it + s2.sweeps;
         st.it64 += s2.it64;
     }
     return done;
