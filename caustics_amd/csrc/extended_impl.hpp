@@ -51,6 +51,7 @@ __host__ __device__ inline size_t ext_ws_bytes(int deg, int mmax) {
     b += ext_align(2 * (size_t)mmax * sizeof(uint32_t));          // flags x2
     b += ext_align((size_t)(mmax + 1) * sizeof(uint64_t));        // permutations / changepoints
     b += ext_align(2 * ((size_t)mmax * deg + 2) * sizeof(double)); // P, Q
+    b += ext_align(2 * ((size_t)mmax * deg + 2) * sizeof(double)); // point cotangents
     return b;
 }
 
@@ -60,6 +61,7 @@ struct ExtWS {
     uint32_t* flg[2];
     uint64_t* perm;
     double* pq;
+    double* xy;
 };
 
 __device__ inline ExtWS ext_ws(char* base, int deg, int mmax) {
@@ -77,6 +79,8 @@ __device__ inline ExtWS ext_ws(char* base, int deg, int mmax) {
     w.perm = (uint64_t*)(base + o);
     o += ext_align((size_t)(mmax + 1) * sizeof(uint64_t));
     w.pq = (double*)(base + o);
+    o += ext_align(2 * ((size_t)mmax * deg + 2) * sizeof(double));
+    w.xy = (double*)(base + o);
     return w;
 }
 
@@ -365,22 +369,48 @@ __device__ inline void gauss_legendre(int n, double* x, double* w) {
     }
 }
 
+// Brightness profile at z (integrate.py:29-44): r = |lens_eq(z) - w0| / rho,
+// B = 3/(3-u1) (u1 B_r + 1 - 2 u1).  Contraction is off here so that the forward-only
+// and the forward+VJP kernels, which inline this into different code, round B alike.
 template <int NL>
-__device__ __forceinline__ double ld_brightness(cd z, const LensFrame<NL>& lf, cd w0, double irho,
-                                                double u1) {
-    cd w = z;
+__device__ __forceinline__ double ld_brightness_core(cd z, const LensFrame<NL>& lf, cd w0,
+                                                     double irho, double u1, cd (&v)[NL], cd& D,
+                                                     double& aD, double& r, double& Br,
+                                                     double& c3) {
+#pragma clang fp contract(off)
     const cd zb = conj(z);
+    cd G = cd{0.0, 0.0};
 #pragma unroll
-    for (int j = 0; j < NL; ++j) w = w - lf.e[j] * crcp<2>(zb - conj(lf.r[j]));
-    const double r = sqrt(norm2(w - w0)) * irho;
-    double Br;
+    for (int j = 0; j < NL; ++j) {
+        v[j] = crcp<2>(zb - conj(lf.r[j]));
+        G = cd{fma(lf.e[j], v[j].re, G.re), fma(lf.e[j], v[j].im, G.im)};  // explicit: no
+    }                                                                       // contraction choice
+    D = z - G - w0;
+    aD = sqrt(norm2(D));
+    r = aD * irho;
     if (r <= 1.0) {
         Br = 1.0 + sqrt(fmax(1.0 - r * r, 0.0));
     } else {
         const double ir = rcp(r);
         Br = 1.0 - sqrt(fmax(1.0 - ir * ir, 0.0));
     }
-    return 3.0 / (3.0 - u1) * (u1 * Br + 1.0 - 2.0 * u1);
+    c3 = 3.0 / (3.0 - u1);
+    return c3 * (u1 * Br + 1.0 - 2.0 * u1);
+}
+
+template <int NL>
+__device__ __forceinline__ double ld_brightness(cd z, const LensFrame<NL>& lf, cd w0, double irho,
+                                                double u1) {
+    cd v[NL], D;
+    double aD, r, Br, c3;
+    return ld_brightness_core<NL>(z, lf, w0, irho, u1, v, D, aD, r, Br, c3);
+}
+
+// one trapezoid term of the limb-darkened contour integral (integrate.py:118-121),
+// written with explicit fma so that both kernels that form it round it alike
+__device__ __forceinline__ double ld_trapezoid(double Pk, double Pn, double Qk, double Qn, cd a,
+                                               cd b, double acc) {
+    return fma(0.5 * (Pk + Pn), b.re - a.re, fma(0.5 * (Qk + Qn), b.im - a.im, acc));
 }
 
 struct GLNodes {
@@ -396,8 +426,9 @@ __device__ __forceinline__ double ld_line(bool isP, double fixed, double a, doub
                                           const GLNodes& g) {
     const double irho = rcp(rho);
     const double d = fabs(b - a);
-    double sp = (b > a) ? b - 2.0 * rho : b + 2.0 * rho;
-    if (0.5 * d <= 2.0 * rho) sp = a + 0.5 * d;
+    // explicit fma: the forward+VJP twin (ld_line_adj) must round the split point alike
+    double sp = (b > a) ? fma(-2.0, rho, b) : fma(2.0, rho, b);
+    if (0.5 * d <= 2.0 * rho) sp = fma(0.5, d, a);
     double I = 0.0;
     for (int part = 0; part < 2; ++part) {
         const double lo = part == 0 ? a : sp, hi = part == 0 ? sp : b;
@@ -406,6 +437,7 @@ __device__ __forceinline__ double ld_line(bool isP, double fixed, double a, doub
         const int n = part == 0 ? g.n1 : g.n2;
         const double h = 0.5 * (hi - lo), c = 0.5 * (hi + lo);
         double s = 0.0;
+#pragma unroll 2
         for (int i = 0; i < n; ++i) {
             const double v = fma(h, x[i], c);
             const cd zz = isP ? cd{fixed, v} : cd{v, fixed};
@@ -456,8 +488,7 @@ __device__ double integrate_contour(const cd* zb, const Piece* pc, int np, int L
             const int k1 = (k + 1 == L) ? 0 : k + 1;
             const cd a = contour_point<DEG>(zb, pc, np, k);
             const cd b = contour_point<DEG>(zb, pc, np, k1);
-            acc += 0.5 * (pq[2 * k] + pq[2 * k1]) * (b.re - a.re) +
-                   0.5 * (pq[2 * k + 1] + pq[2 * k1 + 1]) * (b.im - a.im);
+            acc = ld_trapezoid(pq[2 * k], pq[2 * k1], pq[2 * k + 1], pq[2 * k1 + 1], a, b, acc);
         }
         const double r = block_sum(acc, shd);
         __syncthreads();
@@ -589,132 +620,138 @@ __device__ inline cd contour_locate(const cd* zb, const Piece* pc, int np, int k
     return cd{0.0, 0.0};
 }
 
-// Tangents of the image z (limb angle th) along every direction.
-template <int NL, int ND>
-__device__ inline void image_tangents(cd z, double th, const LensFrame<NL>& lf,
-                                      const ExtTangents<NL>& T, cd (&dz)[ND]) {
-    const ImageTerms<NL> it = image_terms<NL>(z, lf);
-    const double det = 1.0 - norm2(it.K);
-    const double idet = rcp(det);
-    double sn, cs;
-    sincos(th, &sn, &cs);
-#pragma unroll
-    for (int d = 0; d < ND; ++d) {
-        cd u = T.dw0[d] + T.drho[d] * cd{cs, sn};
+// Reverse-mode accumulators of the direct (not through the contour points) dependence
+// of the limb-darkened line integrals on the lens frame, the source centre, rho and u1.
+// A direction with tangents (de_j, dr_j, dw0, drho, du1) receives
+//   sum_j de_j ge_j + Re(gr_j conj(dr_j)) + Re(gw conj(dw0)) + grho drho + gu1 du1,
+// so that each Gauss-Legendre node costs O(NL) instead of O(NL x directions).
+template <int NL>
+struct LDAdj {
+    double ge[NL];
+    cd gr[NL];
+    cd gw;
+    double grho, gu1;
+    __device__ void zero() {
 #pragma unroll
         for (int j = 0; j < NL; ++j) {
-            const cd v = it.inv[j];
-            u = u + T.lt[d].de[j] * v;
-            u = cfma(lf.e[j] * (v * v), conj(T.lt[d].dr[j]), u);
+            ge[j] = 0.0;
+            gr[j] = cd{0.0, 0.0};
         }
-        dz[d] = idet * (u - it.K * conj(u));
+        gw = cd{0.0, 0.0};
+        grho = gu1 = 0.0;
     }
-}
-
-// brightness and its derivatives at z: position (Bx, By) and direct directions
-template <int NL, int ND>
-__device__ inline double ld_brightness_grad(cd z, const LensFrame<NL>& lf, cd w0, double rho,
-                                            double u1, const ExtTangents<NL>& T, double& Bx,
-                                            double& By, double (&Bd)[ND]) {
-    const cd zb = conj(z);
-    cd G = cd{0.0, 0.0}, K = cd{0.0, 0.0};
-    cd v[NL];
+    __device__ void axpy(double s, const LDAdj& o) {
 #pragma unroll
-    for (int j = 0; j < NL; ++j) {
-        v[j] = crcp<2>(zb - conj(lf.r[j]));
-        G = G + lf.e[j] * v[j];
-        K = K + lf.e[j] * (v[j] * v[j]);
+        for (int j = 0; j < NL; ++j) {
+            ge[j] += s * o.ge[j];
+            gr[j] = gr[j] + s * o.gr[j];
+        }
+        gw = gw + s * o.gw;
+        grho += s * o.grho;
+        gu1 += s * o.gu1;
     }
-    const cd D = z - G - w0;
-    const double aD = sqrt(norm2(D));
-    const double irho = rcp(rho);
-    const double r = aD * irho;
-    double Br, dBr;
+    __device__ double along(const LensTangent<NL>& lt, cd dw0, double drho, double du1) const {
+        double v = grho * drho + gu1 * du1 + gw.re * dw0.re + gw.im * dw0.im;
+#pragma unroll
+        for (int j = 0; j < NL; ++j)
+            v += lt.de[j] * ge[j] + gr[j].re * lt.dr[j].re + gr[j].im * lt.dr[j].im;
+        return v;
+    }
+};
+
+// brightness B at z (integrate.py:29-44) and its derivatives along x and y at fixed
+// parameters (Bx, By); c * (direct parameter derivative of B) is added to A.
+// r = |D|/rho with D = lens_eq(z) - w0;  dr = Re(conj(D) dD)/(|D| rho) - r drho/rho,
+// dD = -(dG + dw0),  dG = sum_j de_j v_j + e_j v_j^2 conj(dr_j),  v_j = 1/(conj z - conj r_j).
+template <int NL>
+__device__ inline double ld_brightness_adj(cd z, const LensFrame<NL>& lf, cd w0, double rho,
+                                           double irho, double u1, double c, double& Bx,
+                                           double& By, LDAdj<NL>& A) {
+    cd v[NL], D;
+    double aD, r, Br, c3;
+    const double B = ld_brightness_core<NL>(z, lf, w0, irho, u1, v, D, aD, r, Br, c3);
+    cd K = cd{0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < NL; ++j) K = K + lf.e[j] * (v[j] * v[j]);
+    double dBr;
     if (r <= 1.0) {
         const double x = 1.0 - r * r;
-        Br = 1.0 + sqrt(fmax(x, 0.0));
         dBr = x > 0.0 ? -r / sqrt(x) : 0.0;
     } else {
         const double ir = rcp(r);
         const double x = 1.0 - ir * ir;
-        Br = 1.0 - sqrt(fmax(x, 0.0));
         dBr = x > 0.0 ? -(ir * ir * ir) / sqrt(x) : 0.0;
     }
-    const double c3 = 3.0 / (3.0 - u1);
-    const double B = c3 * (u1 * Br + 1.0 - 2.0 * u1);
     const double dBdr = c3 * u1 * dBr;
     const double iDr = (aD > 0.0) ? rcp(aD * rho) : 0.0;
-    // dr = Re(conj(D) dD)/(|D| rho) - r drho/rho
     const cd dDx = cd{1.0, 0.0} + K;
     const cd dDy = cd{0.0, 1.0} * (cd{1.0, 0.0} - K);
     Bx = dBdr * (D.re * dDx.re + D.im * dDx.im) * iDr;
     By = dBdr * (D.re * dDy.re + D.im * dDy.im) * iDr;
     const double dBdu1 = c3 * c3 / 3.0 * (u1 * Br + 1.0 - 2.0 * u1) + c3 * (Br - 2.0);
+    const double a = c * dBdr * iDr;
+    const cd Dc = conj(D);
+    A.gw = A.gw - a * D;
 #pragma unroll
-    for (int d = 0; d < ND; ++d) {
-        cd dG = cd{0.0, 0.0};
-#pragma unroll
-        for (int j = 0; j < NL; ++j) {
-            dG = dG + T.lt[d].de[j] * v[j];
-            dG = cfma(lf.e[j] * (v[j] * v[j]), conj(T.lt[d].dr[j]), dG);
-        }
-        const cd dD = -1.0 * (dG + T.dw0[d]);
-        const double dr = (D.re * dD.re + D.im * dD.im) * iDr - r * T.drho[d] * irho;
-        Bd[d] = dBdr * dr + dBdu1 * T.du1[d];
+    for (int j = 0; j < NL; ++j) {
+        A.ge[j] -= a * (D.re * v[j].re + D.im * v[j].im);
+        A.gr[j] = A.gr[j] - (a * lf.e[j]) * (Dc * (v[j] * v[j]));
     }
+    A.grho -= c * dBdr * r * irho;
+    A.gu1 += c * dBdu1;
     return B;
 }
 
-// P (isP) or Q line integral with its partial derivatives: d/d(fixed coordinate),
-// d/d(lower limit a), d/d(upper limit b), and direct directions.
-template <int NL, int ND>
-__device__ inline double ld_line_grad(bool isP, double fixed, double a, double b, double rho,
-                                      const LensFrame<NL>& lf, cd w0, double u1,
-                                      const GLNodes& g, const ExtTangents<NL>& T, double& dfix,
-                                      double& da, double& db, double (&dth)[ND]) {
+// P (isP) or Q line integral with its partial derivatives along the fixed coordinate,
+// the lower limit a and the upper limit b; wA * (direct parameter derivative) goes to A.
+template <int NL>
+__device__ inline double ld_line_adj(bool isP, double fixed, double a, double b, double rho,
+                                     const LensFrame<NL>& lf, cd w0, double u1, const GLNodes& g,
+                                     double wA, double& dfix, double& da, double& db,
+                                     LDAdj<NL>& A) {
     const double d = fabs(b - a);
+    const double irho = rcp(rho);
+    const double sc = isP ? -0.5 : 0.5;
     double sp, dsp_da, dsp_db, dsp_drho;
     if (0.5 * d <= 2.0 * rho) {
         const double sg = (b - a) > 0 ? 1.0 : ((b - a) < 0 ? -1.0 : 0.0);
-        sp = a + 0.5 * d;
+        sp = fma(0.5, d, a);
         dsp_da = 1.0 - 0.5 * sg;
         dsp_db = 0.5 * sg;
         dsp_drho = 0.0;
     } else if (b > a) {
-        sp = b - 2.0 * rho;
+        sp = fma(-2.0, rho, b);
         dsp_da = 0.0;
         dsp_db = 1.0;
         dsp_drho = -2.0;
     } else {
-        sp = b + 2.0 * rho;
+        sp = fma(2.0, rho, b);
         dsp_da = 0.0;
         dsp_db = 1.0;
         dsp_drho = 2.0;
     }
     double S = 0.0, dS_lo[2], dS_hi[2];
     dfix = 0.0;
-#pragma unroll
-    for (int q = 0; q < ND; ++q) dth[q] = 0.0;
     for (int part = 0; part < 2; ++part) {
         const double lo = part == 0 ? a : sp, hi = part == 0 ? sp : b;
         const double* x = part == 0 ? g.x1 : g.x2;
         const double* wt = part == 0 ? g.w1 : g.w2;
         const int n = part == 0 ? g.n1 : g.n2;
         const double h = 0.5 * (hi - lo), c = 0.5 * (hi + lo);
+        const double cw = wA * sc * h;
         double s = 0.0, slo = 0.0, shi = 0.0;
+#pragma unroll 1
         for (int i = 0; i < n; ++i) {
             const double t = fma(h, x[i], c);
             const cd zz = isP ? cd{fixed, t} : cd{t, fixed};
-            double Bx, By, Bd[ND];
-            const double B = ld_brightness_grad<NL, ND>(zz, lf, w0, rho, u1, T, Bx, By, Bd);
+            double Bx, By;
+            const double B = ld_brightness_adj<NL>(zz, lf, w0, rho, irho, u1, cw * wt[i], Bx, By, A);
             const double fl = isP ? By : Bx;   // along the line
             const double ff = isP ? Bx : By;   // across (fixed coordinate)
             s = fma(h * B, wt[i], s);
             slo += wt[i] * (-0.5 * B + 0.5 * h * fl * (1.0 - x[i]));
             shi += wt[i] * (0.5 * B + 0.5 * h * fl * (1.0 + x[i]));
             dfix += h * wt[i] * ff;
-#pragma unroll
-            for (int q = 0; q < ND; ++q) dth[q] += h * wt[i] * Bd[q];
         }
         S += s;
         dS_lo[part] = slo;
@@ -723,25 +760,48 @@ __device__ inline double ld_line_grad(bool isP, double fixed, double a, double b
     const double dsp = dS_hi[0] + dS_lo[1];
     da = dS_lo[0] + dsp * dsp_da;
     db = dS_hi[1] + dsp * dsp_db;
-#pragma unroll
-    for (int q = 0; q < ND; ++q) dth[q] += dsp * dsp_drho * T.drho[q];
-    const double sc = isP ? -0.5 : 0.5;
+    A.grho += wA * sc * dsp * dsp_drho;
     dfix *= sc;
     da *= sc;
     db *= sc;
-#pragma unroll
-    for (int q = 0; q < ND; ++q) dth[q] *= sc;
     return sc * S;
 }
 
 // Adjoint of one contour integral: accumulates scale * dI/dtheta into g[ND] (per thread).
 // pq holds P, Q of the contour points (limb darkening; written by the forward pass).
-template <int NL, int DEG, bool LD, int ND>
+// Adjoint of the image z of limb angle th for the cotangent zbar = (bx, by) of z:
+// dz = (u - K conj u)/det with u = dw0 + drho e^{i th} + sum_j de_j v_j + e_j v_j^2 conj(dr_j)
+// gives zbar . dz = Re(c u), c = (conj(zbar) - zbar conj(K))/det, added to A.
+template <int NL>
+__device__ inline void image_adjoint(cd z, double th, const LensFrame<NL>& lf, double bx,
+                                     double by, LDAdj<NL>& A) {
+    const ImageTerms<NL> it = image_terms<NL>(z, lf);
+    const double idet = rcp(1.0 - norm2(it.K));
+    const cd zbar = cd{bx, by};
+    const cd c = idet * (conj(zbar) - zbar * conj(it.K));
+    double sn, cs;
+    sincos(th, &sn, &cs);
+    A.gw = A.gw + conj(c);
+    A.grho += c.re * cs - c.im * sn;
+#pragma unroll
+    for (int j = 0; j < NL; ++j) {
+        const cd v = it.inv[j];
+        const cd cv = c * v;
+        A.ge[j] += cv.re;
+        A.gr[j] = A.gr[j] + lf.e[j] * (cv * v);
+    }
+}
+
+// Adjoint of one contour integral: adds scale * dI/d(frame, w0, rho, u1) to adj (per
+// thread).  pq holds P, Q of the contour points (limb darkening; written by the forward
+// pass).  Pass 1 forms each point's cotangent (and the direct dependence of its P, Q
+// lines), stored in xy; the centroid's cotangent is known only after a block reduction,
+// so pass 2 adds it and pushes every point's cotangent through the image tangents.
+template <int NL, int DEG, bool LD>
 __device__ void contour_adjoint(const cd* zb, const double* theta, const Piece* pc, int np, int L,
-                                int nterms, double cdiv, const double* pq,
+                                int nterms, double cdiv, const double* pq, double* xy,
                                 const LensFrame<NL>& lf, cd w0, double rho, double u1,
-                                const GLNodes& gl, const ExtTangents<NL>& T, double scale,
-                                double (&g)[ND], double* shd) {
+                                const GLNodes& gl, double scale, LDAdj<NL>& adj, double* shd) {
     const int tid = threadIdx.x;
     if (L <= 0) return;
     double zx = 0.0, zy = 0.0;
@@ -757,12 +817,8 @@ __device__ void contour_adjoint(const cd* zb, const double* theta, const Piece* 
         zy = (block_sum(sy, shd) + c0.im) / cdiv;
     }
     double x0bar = 0.0, y0bar = 0.0;   // adjoints of the centroid (per thread)
-    cd sdz[ND];                        // sum of dz over points, weighted for the centroid
-#pragma unroll
-    for (int d = 0; d < ND; ++d) sdz[d] = cd{0.0, 0.0};
     for (int k = tid; k < L; k += kExtBlock) {
-        int idx, row;
-        const cd zk = contour_locate<DEG>(zb, pc, np, k, idx, row);
+        const cd zk = contour_point<DEG>(zb, pc, np, k);
         const int kn = (k + 1 == L) ? 0 : k + 1;
         const int kp = (k == 0) ? L - 1 : k - 1;
         const bool t_first = k < nterms;    // term k: (k, kn)
@@ -770,9 +826,6 @@ __device__ void contour_adjoint(const cd* zb, const double* theta, const Piece* 
         const cd zn = contour_point<DEG>(zb, pc, np, kn);
         const cd zp = contour_point<DEG>(zb, pc, np, kp);
         double xbar = 0.0, ybar = 0.0;
-        double thd[ND];
-#pragma unroll
-        for (int d = 0; d < ND; ++d) thd[d] = 0.0;
         if constexpr (!LD) {
             if (t_first) {
                 xbar += 0.5 * zn.im;
@@ -799,37 +852,125 @@ __device__ void contour_adjoint(const cd* zb, const double* theta, const Piece* 
                 xbar += 0.5 * (Pp + Pk);
                 ybar += 0.5 * (Qp + Qk);
             }
-            double dfix, da, db, dth[ND];
+            double dfix, da, db;
             // P_k = P(x_k; y0 .. y_k)
-            ld_line_grad<NL, ND>(true, zk.re, zy, zk.im, rho, lf, w0, u1, gl, T, dfix, da, db, dth);
+            ld_line_adj<NL>(true, zk.re, zy, zk.im, rho, lf, w0, u1, gl, scale * Pbar, dfix, da,
+                            db, adj);
             xbar += Pbar * dfix;
             ybar += Pbar * db;
             y0bar += Pbar * da;
-#pragma unroll
-            for (int d = 0; d < ND; ++d) thd[d] += Pbar * dth[d];
             // Q_k = Q(y_k; x0 .. x_k)
-            ld_line_grad<NL, ND>(false, zk.im, zx, zk.re, rho, lf, w0, u1, gl, T, dfix, da, db, dth);
+            ld_line_adj<NL>(false, zk.im, zx, zk.re, rho, lf, w0, u1, gl, scale * Qbar, dfix, da,
+                            db, adj);
             ybar += Qbar * dfix;
             xbar += Qbar * db;
             x0bar += Qbar * da;
-#pragma unroll
-            for (int d = 0; d < ND; ++d) thd[d] += Qbar * dth[d];
         }
-        cd dz[ND];
-        image_tangents<NL, ND>(zk, theta[idx], lf, T, dz);
-        const double wgt = (k == 0) ? 2.0 : 1.0;  // the closing point repeats c_0 in the centroid
-#pragma unroll
-        for (int d = 0; d < ND; ++d) {
-            g[d] += scale * (xbar * dz[d].re + ybar * dz[d].im + thd[d]);
-            sdz[d] = sdz[d] + wgt * dz[d];
-        }
+        xy[2 * k] = xbar;
+        xy[2 * k + 1] = ybar;
     }
+    double X0 = 0.0, Y0 = 0.0;
     if constexpr (LD) {
-        const double X0 = block_sum(x0bar, shd) / cdiv;
-        const double Y0 = block_sum(y0bar, shd) / cdiv;
-#pragma unroll
-        for (int d = 0; d < ND; ++d) g[d] += scale * (X0 * sdz[d].re + Y0 * sdz[d].im);
+        X0 = block_sum(x0bar, shd) / cdiv;
+        Y0 = block_sum(y0bar, shd) / cdiv;
     }
+    __syncthreads();
+    for (int k = tid; k < L; k += kExtBlock) {
+        int idx, row;
+        const cd zk = contour_locate<DEG>(zb, pc, np, k, idx, row);
+        const double wgt = (k == 0) ? 2.0 : 1.0;  // the closing point repeats c_0 in the centroid
+        image_adjoint<NL>(zk, theta[idx], lf, scale * (xy[2 * k] + wgt * X0),
+                          scale * (xy[2 * k + 1] + wgt * Y0), adj);
+    }
+    __syncthreads();
+}
+
+// Limb-darkened contour integral and its adjoint in one sweep over the Gauss-Legendre
+// nodes (forward + VJP): pass A evaluates every point's P, Q lines with their derivatives
+// and the direct parameter adjoint (the line cotangents Pbar, Qbar depend on the point
+// geometry only); pass B forms the trapezoid sum and the points' cotangents; pass C
+// pushes those through the image tangents.  Returns the integral, bit-identical to
+// integrate_contour<..., LD=true>.
+template <int NL, int DEG>
+__device__ double contour_ld_fused(const cd* zb, const double* theta, const Piece* pc, int np,
+                                   int L, int nterms, double cdiv, double* pq, double* xy,
+                                   const LensFrame<NL>& lf, cd w0, double rho, double u1,
+                                   const GLNodes& gl, double scale, LDAdj<NL>& adj, double* shd) {
+    const int tid = threadIdx.x;
+    if (L <= 0) return block_sum(0.0, shd);
+    double sx = 0.0, sy = 0.0;
+    for (int k = tid; k < L; k += kExtBlock) {
+        const cd a = contour_point<DEG>(zb, pc, np, k);
+        sx += a.re;
+        sy += a.im;
+    }
+    const cd c0 = contour_point<DEG>(zb, pc, np, 0);
+    const double zx = (block_sum(sx, shd) + c0.re) / cdiv;
+    const double zy = (block_sum(sy, shd) + c0.im) / cdiv;
+    double x0bar = 0.0, y0bar = 0.0;
+    // pass A
+    for (int k = tid; k < L; k += kExtBlock) {
+        const cd zk = contour_point<DEG>(zb, pc, np, k);
+        const int kn = (k + 1 == L) ? 0 : k + 1;
+        const int kp = (k == 0) ? L - 1 : k - 1;
+        const cd zn = contour_point<DEG>(zb, pc, np, kn);
+        const cd zp = contour_point<DEG>(zb, pc, np, kp);
+        double Pbar = 0.0, Qbar = 0.0;
+        if (k < nterms) {
+            Pbar += 0.5 * (zn.re - zk.re);
+            Qbar += 0.5 * (zn.im - zk.im);
+        }
+        if (kp < nterms) {
+            Pbar += 0.5 * (zk.re - zp.re);
+            Qbar += 0.5 * (zk.im - zp.im);
+        }
+        double dfP, daP, dbP, dfQ, daQ, dbQ;
+        pq[2 * k] = ld_line_adj<NL>(true, zk.re, zy, zk.im, rho, lf, w0, u1, gl, scale * Pbar,
+                                    dfP, daP, dbP, adj);
+        pq[2 * k + 1] = ld_line_adj<NL>(false, zk.im, zx, zk.re, rho, lf, w0, u1, gl,
+                                        scale * Qbar, dfQ, daQ, dbQ, adj);
+        xy[2 * k] = Pbar * dfP + Qbar * dbQ;
+        xy[2 * k + 1] = Pbar * dbP + Qbar * dfQ;
+        y0bar += Pbar * daP;
+        x0bar += Qbar * daQ;
+    }
+    __syncthreads();
+    // pass B
+    double acc = 0.0;
+    for (int k = tid; k < L; k += kExtBlock) {
+        const int kn = (k + 1 == L) ? 0 : k + 1;
+        const int kp = (k == 0) ? L - 1 : k - 1;
+        const double Pk = pq[2 * k], Qk = pq[2 * k + 1];
+        const double Pn = pq[2 * kn], Qn = pq[2 * kn + 1];
+        const double Pp = pq[2 * kp], Qp = pq[2 * kp + 1];
+        double xbar = xy[2 * k], ybar = xy[2 * k + 1];
+        if (k < nterms) {
+            const cd a = contour_point<DEG>(zb, pc, np, k);
+            const cd b = contour_point<DEG>(zb, pc, np, kn);
+            acc = ld_trapezoid(Pk, Pn, Qk, Qn, a, b, acc);
+            xbar -= 0.5 * (Pk + Pn);
+            ybar -= 0.5 * (Qk + Qn);
+        }
+        if (kp < nterms) {
+            xbar += 0.5 * (Pp + Pk);
+            ybar += 0.5 * (Qp + Qk);
+        }
+        xy[2 * k] = xbar;
+        xy[2 * k + 1] = ybar;
+    }
+    const double I = block_sum(acc, shd);
+    const double X0 = block_sum(x0bar, shd) / cdiv;
+    const double Y0 = block_sum(y0bar, shd) / cdiv;
+    // pass C
+    for (int k = tid; k < L; k += kExtBlock) {
+        int idx, row;
+        const cd zk = contour_locate<DEG>(zb, pc, np, k, idx, row);
+        const double wgt = (k == 0) ? 2.0 : 1.0;  // the closing point repeats c_0 in the centroid
+        image_adjoint<NL>(zk, theta[idx], lf, scale * (xy[2 * k] + wgt * X0),
+                          scale * (xy[2 * k + 1] + wgt * Y0), adj);
+    }
+    __syncthreads();
+    return I;
 }
 
 }  // namespace caus
@@ -1442,7 +1583,8 @@ __global__ __launch_bounds__(kExtBlock) void k_ext_contours(ExtArgs A, int buf) 
 // ---- contour integrals, magnification and its reverse mode (extended_source.py:
 // 860-901; integrate.py:18-121)
 template <int NL, bool LD, bool VJP>
-__global__ __launch_bounds__(kExtBlock) void k_ext_integrate(ExtArgs A, int buf) {
+__global__ __launch_bounds__(kExtBlock) __attribute__((amdgpu_waves_per_eu(VJP ? 2 : 3)))
+void k_ext_integrate(ExtArgs A, int buf) {
     constexpr int DEG = ext_degree<NL>();
     constexpr int ND = ext_nd<NL>();
     __shared__ double s_d[8];
@@ -1476,17 +1618,23 @@ __global__ __launch_bounds__(kExtBlock) void k_ext_integrate(ExtArgs A, int buf)
     double gacc[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) gacc[d] = 0.0;
-    ExtTangents<NL> T;
-    if constexpr (VJP) ext_tangents<NL>(A.params + src * A.pstride, T);
+    LDAdj<NL> adj;
+    adj.zero();
     unsigned long long npoints = 0;
     auto do_contour = [&](const Piece* pcs, int npc, int L, int nterms, double cdiv, int par) {
         npoints += (L > 0) ? (unsigned long long)L : 0ull;
-        const double I = integrate_contour<NL, DEG, LD>(zb, pcs, npc, L, nterms, cdiv, W.pq, lf,
-                                                        w0, rho, u1, g, s_d);
-        total += par * I;
-        if constexpr (VJP)
-            contour_adjoint<NL, DEG, LD, ND>(zb, thb, pcs, npc, L, nterms, cdiv, W.pq, lf, w0, rho,
-                                             u1, g, T, (double)par, gacc, s_d);
+        if constexpr (LD && VJP) {
+            total += par * contour_ld_fused<NL, DEG>(zb, thb, pcs, npc, L, nterms, cdiv, W.pq,
+                                                     W.xy, lf, w0, rho, u1, g, (double)par, adj,
+                                                     s_d);
+        } else {
+            const double I = integrate_contour<NL, DEG, LD>(zb, pcs, npc, L, nterms, cdiv, W.pq,
+                                                            lf, w0, rho, u1, g, s_d);
+            total += par * I;
+            if constexpr (VJP)
+                contour_adjoint<NL, DEG, LD>(zb, thb, pcs, npc, L, nterms, cdiv, W.pq, W.xy, lf,
+                                             w0, rho, u1, g, (double)par, adj, s_d);
+        }
     };
     if constexpr (NL == 1) {
         // both images, tail index N-1 (extended_source.py:860-866)
@@ -1517,6 +1665,10 @@ __global__ __launch_bounds__(kExtBlock) void k_ext_integrate(ExtArgs A, int buf)
     if constexpr (VJP) {
         const double base = (total >= 0.0 ? 1.0 : -1.0) / (kExtPi * rho * rho);
         const double gm = A.g_mag[src];
+        ExtTangents<NL> T;
+        ext_tangents<NL>(A.params + src * A.pstride, T);
+#pragma unroll
+        for (int d = 0; d < ND; ++d) gacc[d] = adj.along(T.lt[d], T.dw0[d], T.drho[d], T.du1[d]);
 #pragma unroll
         for (int d = 0; d < ND; ++d) {
             double v = block_sum(gacc[d], s_d) * base;
