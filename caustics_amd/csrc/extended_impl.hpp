@@ -307,6 +307,7 @@ struct Seg {
     int par;              // parity of the segment (+1/-1)
     int empty;
     double length;        // sum of |z_{k+1} - z_k|
+    cd head, tail;        // first and last point (0 for an empty segment)
 };
 struct Piece {
     int row, start, len, rev;
@@ -1360,9 +1361,14 @@ __global__ __launch_bounds__(kExtBlock) void k_ext_contours(ExtArgs A, int buf) 
     constexpr int DEG = ext_degree<NL>();
     constexpr int NIMG = NL * NL + 1;
     constexpr int NSEG = 3 * NIMG;  // nr_of_segments
+    constexpr int NRUN = 10;        // starts / ends kept per row (the reference's split)
     __shared__ ActSeg s_con[3];
+    __shared__ ActSeg s_act;
+    __shared__ Seg s_cand[kExtMaxSeg];
+    __shared__ int s_st[DEG][NRUN], s_en[DEG][NRUN], s_nst[DEG], s_nen[DEG];
     __shared__ unsigned s_allmask;
     const int tid = threadIdx.x;
+    const int lane = tid & 63;
     const int64_t src = blockIdx.x;
     const ExtBatch B = ext_batch(A);
     const int M = ext_points(B.npts[src], 10);
@@ -1372,6 +1378,7 @@ __global__ __launch_bounds__(kExtBlock) void k_ext_contours(ExtArgs A, int buf) 
     const uint32_t* fb = W.flg[buf];
     // closure test (extended_source.py:289-291)
     if (tid == 0) s_allmask = 0xFFFFu;
+    if (tid < DEG) s_nst[tid] = s_nen[tid] = 0;
     __syncthreads();
     unsigned am = 0xFFFFu;
     for (int k = tid; k < M; k += kExtBlock) am &= fb[k];
@@ -1384,197 +1391,259 @@ __global__ __launch_bounds__(kExtBlock) void k_ext_contours(ExtArgs A, int buf) 
             if (norm2(pt<DEG>(zb, 0, r) - pt<DEG>(zb, M - 1, r)) < 1e-10) closed |= 1u << r;
         }
     }
-    if (closed != (1u << DEG) - 1u) {
-            // changepoints per boundary c in [0, M] (extended_source.py:163-188); rows
-            // that are closed contribute nothing (their segments were zeroed)
-            const unsigned open = ((1u << DEG) - 1u) & ~closed;
-            uint64_t* cpw = W.perm;
-            for (int c = tid; c <= M; c += kExtBlock) {
-                uint64_t word = 0;
+    const bool all_closed = closed == (1u << DEG) - 1u;
+    if (!all_closed) {
+        // changepoints per boundary c in [0, M] (extended_source.py:163-188); rows that
+        // are closed contribute nothing (their segments were zeroed)
+        const unsigned open = ((1u << DEG) - 1u) & ~closed;
+        uint64_t* cpw = W.perm;
+        for (int c = tid; c <= M; c += kExtBlock) {
+            uint64_t word = 0;
 #pragma unroll
-                for (int r = 0; r < DEG; ++r) {
-                    if (!((open >> r) & 1u)) continue;
-                    const int mp = (c > 0) ? (int)((fb[c - 1] >> r) & 1u) : 0;
-                    const int mc = (c < M) ? (int)((fb[c] >> r) & 1u) : 0;
-                    bool cp;
-                    int dm;
-                    if (c == 0) {
-                        cp = mc != 0;
-                        dm = mc ? 1 : 0;
-                    } else if (c == M) {
-                        cp = mp != 0;
-                        dm = mp ? -1 : 0;
-                    } else {
-                        const cd za = mp ? pt<DEG>(zb, c - 1, r) : cd{0.0, 0.0};
-                        const cd zc = mc ? pt<DEG>(zb, c, r) : cd{0.0, 0.0};
-                        const int pa = mp ? ((((fb[c - 1] >> (16 + r)) & 1u) ? -1 : 1)) : 0;
-                        const int pc_ = mc ? ((((fb[c] >> (16 + r)) & 1u) ? -1 : 1)) : 0;
-                        dm = mc - mp;
-                        cp = (norm2(zc - za) > 0.01) || (pc_ != pa) || (dm != 0);
-                    }
-                    if (cp && dm >= 0) word |= 1ull << r;
-                    if (cp && dm <= 0) word |= 1ull << (16 + r);
+            for (int r = 0; r < DEG; ++r) {
+                if (!((open >> r) & 1u)) continue;
+                const int mp = (c > 0) ? (int)((fb[c - 1] >> r) & 1u) : 0;
+                const int mc = (c < M) ? (int)((fb[c] >> r) & 1u) : 0;
+                bool cp;
+                int dm;
+                if (c == 0) {
+                    cp = mc != 0;
+                    dm = mc ? 1 : 0;
+                } else if (c == M) {
+                    cp = mp != 0;
+                    dm = mp ? -1 : 0;
+                } else {
+                    const cd za = mp ? pt<DEG>(zb, c - 1, r) : cd{0.0, 0.0};
+                    const cd zc = mc ? pt<DEG>(zb, c, r) : cd{0.0, 0.0};
+                    const int pa = mp ? ((((fb[c - 1] >> (16 + r)) & 1u) ? -1 : 1)) : 0;
+                    const int pc_ = mc ? ((((fb[c] >> (16 + r)) & 1u) ? -1 : 1)) : 0;
+                    dm = mc - mp;
+                    cp = (norm2(zc - za) > 0.01) || (pc_ != pa) || (dm != 0);
                 }
-                cpw[c] = word;
+                if (cp && dm >= 0) word |= 1ull << r;
+                if (cp && dm <= 0) word |= 1ull << (16 + r);
             }
-            __syncthreads();
-            if (tid < 64) {
-                // runs per row (first 10 starts / ends), candidate index row*10+slot;
-                // nonempty (>= 2 points) ones in descending index order (:222-240)
-                Seg cand[kExtMaxSeg];
-                int nc = 0;
-                for (int r = DEG - 1; r >= 0 && nc < NSEG; --r) {
-                    if (!((open >> r) & 1u)) continue;
-                    int st[10], en[10], ns = 0, ne = 0;
-                    for (int c = 0; c <= M; ++c) {
-                        const uint64_t wd = cpw[c];
-                        if (((wd >> r) & 1ull) && ns < 10) st[ns++] = c;
-                        if (((wd >> (16 + r)) & 1ull) && ne < 10) en[ne++] = c;
+            cpw[c] = word;
+        }
+        __syncthreads();
+        if (tid < 64) {
+            // the first NRUN starts and ends of every row, by ballots over 64 boundaries
+            for (int base = 0; base <= M; base += 64) {
+                const int c = base + lane;
+                const uint64_t wd = (c <= M) ? cpw[c] : 0ull;
+                const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+                for (int r = 0; r < DEG; ++r) {
+                    const uint64_t ms = __ballot((wd >> r) & 1ull);
+                    const uint64_t me = __ballot((wd >> (16 + r)) & 1ull);
+                    const int ns = s_nst[r], ne = s_nen[r];
+                    if ((wd >> r) & 1ull) {
+                        const int rank = ns + __popcll(ms & below);
+                        if (rank < NRUN) s_st[r][rank] = c;
                     }
-                    for (int q = 9; q >= 0 && nc < NSEG; --q) {
-                        const int l = q < ns ? st[q] : 0, rr = q < ne ? en[q] : 0;
-                        if (l == 0 && rr == 0) continue;
-                        if (rr - l < 2) continue;
-                        Seg s;
-                        s.row = r;
-                        s.start = l;
-                        s.len = rr - l;
-                        s.par = ((fb[l] >> (16 + r)) & 1u) ? -1 : 1;
-                        s.empty = 0;
-                        double L = 0.0;
-                        for (int k = 0; k + 1 < s.len; ++k)
-                            L += sqrt(norm2(pt<DEG>(zb, l + k + 1, r) - pt<DEG>(zb, l + k, r)));
-                        s.length = L;
-                        cand[nc++] = s;
+                    if ((wd >> (16 + r)) & 1ull) {
+                        const int rank = ne + __popcll(me & below);
+                        if (rank < NRUN) s_en[r][rank] = c;
                     }
+                    __builtin_amdgcn_wave_barrier();
+                    if (lane == 0) {
+                        s_nst[r] = min(NRUN, ns + __popcll(ms));
+                        s_nen[r] = min(NRUN, ne + __popcll(me));
+                    }
+                    __builtin_amdgcn_wave_barrier();
                 }
-                for (int q = nc; q < NSEG; ++q) {
-                    cand[q].empty = 1;
-                    cand[q].len = 0;
-                    cand[q].length = 0.0;
-                    cand[q].par = 0;
-                    cand[q].row = 0;
-                    cand[q].start = 0;
+            }
+            // candidate index row*10+slot; nonempty (>= 2 points) ones in descending
+            // index order (:222-240); wave-uniform, every lane writes the same values
+            int nc = 0;
+            for (int r = DEG - 1; r >= 0 && nc < NSEG; --r) {
+                if (!((open >> r) & 1u)) continue;
+                const int ns = s_nst[r], ne = s_nen[r];
+                for (int q = NRUN - 1; q >= 0 && nc < NSEG; --q) {
+                    const int l = q < ns ? s_st[r][q] : 0, rr = q < ne ? s_en[r][q] : 0;
+                    if (l == 0 && rr == 0) continue;
+                    if (rr - l < 2) continue;
+                    Seg sg;
+                    sg.row = r;
+                    sg.start = l;
+                    sg.len = rr - l;
+                    sg.par = ((fb[l] >> (16 + r)) & 1u) ? -1 : 1;
+                    sg.empty = 0;
+                    sg.length = 0.0;
+                    sg.head = pt<DEG>(zb, l, r);
+                    sg.tail = pt<DEG>(zb, rr - 1, r);
+                    s_cand[nc] = sg;
+                    ++nc;
                 }
-                // stitching, 3 contours, merge scans of 20, 18, 16 (:495-667, :690)
-                int R = NSEG;
-                int maxm = 20;
-                for (int ci = 0; ci < 3; ++ci) {
-                    // stable sort by length, empties (length 0) last
-                    for (int a = 1; a < R; ++a) {
-                        Seg key = cand[a];
-                        int b = a - 1;
-                        auto later = [](const Seg& x, const Seg& y) {  // x after y ?
-                            const bool xe = x.empty || x.length == 0.0;
-                            const bool ye = y.empty || y.length == 0.0;
-                            if (xe != ye) return xe;
-                            if (xe) return false;
-                            return x.length > y.length;
-                        };
-                        while (b >= 0 && later(cand[b], key)) {
-                            cand[b + 1] = cand[b];
-                            --b;
-                        }
-                        cand[b + 1] = key;
+            }
+            for (int q = nc; q < NSEG; ++q) {
+                Seg sg;
+                sg.empty = 1;
+                sg.len = 0;
+                sg.length = 0.0;
+                sg.par = 0;
+                sg.row = 0;
+                sg.start = 0;
+                sg.head = sg.tail = cd{0.0, 0.0};
+                s_cand[q] = sg;
+            }
+            __builtin_amdgcn_wave_barrier();
+            // segment lengths, lanes over points
+            for (int q = 0; q < nc; ++q) {
+                const Seg sg = s_cand[q];
+                double L = 0.0;
+                for (int k = lane; k + 1 < sg.len; k += 64)
+                    L += sqrt(norm2(pt<DEG>(zb, sg.start + k + 1, sg.row) -
+                                    pt<DEG>(zb, sg.start + k, sg.row)));
+                L = wave_sum(L);
+                __builtin_amdgcn_wave_barrier();
+                if (lane == 0) s_cand[q].length = L;
+                __builtin_amdgcn_wave_barrier();
+            }
+            // stitching, 3 contours, merge scans of 20, 18, 16 (:495-667, :690)
+            Seg* cand = s_cand;
+            ActSeg& act = s_act;
+            int R = NSEG;
+            int maxm = 20;
+            for (int ci = 0; ci < 3; ++ci) {
+                // stable sort by length, empties (length 0) last
+                for (int a = 1; a < R; ++a) {
+                    const Seg key = cand[a];
+                    int b = a - 1;
+                    auto later = [](const Seg& x, const Seg& y) {  // x after y ?
+                        const bool xe = x.empty || x.length == 0.0;
+                        const bool ye = y.empty || y.length == 0.0;
+                        if (xe != ye) return xe;
+                        if (xe) return false;
+                        return x.length > y.length;
+                    };
+                    while (b >= 0 && later(cand[b], key)) {
+                        const Seg t = cand[b];
+                        __builtin_amdgcn_wave_barrier();
+                        cand[b + 1] = t;
+                        __builtin_amdgcn_wave_barrier();
+                        --b;
                     }
-                    ActSeg act;
-                    act.empty = cand[0].empty;
+                    __builtin_amdgcn_wave_barrier();
+                    cand[b + 1] = key;
+                    __builtin_amdgcn_wave_barrier();
+                }
+                const Seg c0 = cand[0];
+                __builtin_amdgcn_wave_barrier();
+                if (lane == 0) {
+                    act.empty = c0.empty;
                     act.np = 1;
-                    act.pc[0] = Piece{cand[0].row, cand[0].start, cand[0].len, 0};
-                    act.t = cand[0].empty ? 0 : cand[0].len - 1;
-                    act.par = cand[0].par;
-                    for (int q = 1; q < R; ++q) cand[q - 1] = cand[q];
-                    R -= 1;
-                    for (int it = 0; it < maxm; ++it) {
-                        bool any = false;
-                        for (int q = 0; q < R; ++q) any = any || !cand[q].empty;
-                        if (!any) break;
-                        auto ag = [&](int k) { return act_point<DEG>(zb, act, k); };
-                        const cd ah = ag(0), at = ag(act.t);
-                        // 4 nearest (ctype-major flat index, stable)
-                        int bi[4];
-                        double bd[4];
-                        int nb = 0;
-                        for (int ct = 0; ct < 4; ++ct) {
-                            for (int q = 0; q < R; ++q) {
-                                const Seg& s = cand[q];
-                                const cd sh = seg_point<DEG>(zb, s, 0);
-                                const cd stl = seg_point<DEG>(zb, s, s.empty ? 0 : s.len - 1);
-                                const cd d = ct == 0 ? at - sh : ct == 1 ? ah - stl
-                                                               : ct == 2 ? ah - sh : at - stl;
-                                const double dd = sqrt(norm2(d));
-                                const int f = ct * R + q;
-                                int pos = nb;
-                                while (pos > 0 && bd[pos - 1] > dd) --pos;
-                                if (pos < 4) {
-                                    for (int u = min(nb, 3); u > pos; --u) {
-                                        bd[u] = bd[u - 1];
-                                        bi[u] = bi[u - 1];
-                                    }
-                                    bd[pos] = dd;
-                                    bi[pos] = f;
-                                    if (nb < 4) ++nb;
+                    act.pc[0] = Piece{c0.row, c0.start, c0.len, 0};
+                    act.t = c0.empty ? 0 : c0.len - 1;
+                    act.par = c0.par;
+                }
+                cd ah = c0.head, at = c0.tail;  // act's first and last point
+                for (int q = 1; q < R; ++q) {
+                    const Seg t = cand[q];
+                    __builtin_amdgcn_wave_barrier();
+                    cand[q - 1] = t;
+                    __builtin_amdgcn_wave_barrier();
+                }
+                R -= 1;
+                __builtin_amdgcn_wave_barrier();
+                for (int it = 0; it < maxm; ++it) {
+                    bool any = false;
+                    for (int q = 0; q < R; ++q) any = any || !cand[q].empty;
+                    if (!any) break;
+                    auto ag = [&](int k) { return act_point<DEG>(zb, act, k); };
+                    // 4 nearest (ctype-major flat index, stable)
+                    int bi[4];
+                    double bd[4];
+                    int nb = 0;
+                    for (int ct = 0; ct < 4; ++ct) {
+                        for (int q = 0; q < R; ++q) {
+                            const cd sh = cand[q].head, stl = cand[q].tail;
+                            const cd d = ct == 0 ? at - sh : ct == 1 ? ah - stl
+                                                           : ct == 2 ? ah - sh : at - stl;
+                            const double dd = sqrt(norm2(d));
+                            const int f = ct * R + q;
+                            int pos = nb;
+                            while (pos > 0 && bd[pos - 1] > dd) --pos;
+                            if (pos < 4) {
+                                for (int u = min(nb, 3); u > pos; --u) {
+                                    bd[u] = bd[u - 1];
+                                    bi[u] = bi[u - 1];
                                 }
+                                bd[pos] = dd;
+                                bi[pos] = f;
+                                if (nb < 4) ++nb;
                             }
                         }
-                        int chosen = -1;
-                        for (int u = 0; u < nb && chosen < 0; ++u) {
-                            const int ct = bi[u] / R, q = bi[u] % R;
-                            const Seg& s = cand[q];
-                            if (s.empty) continue;  // zero segments connect only within 1e-5 of 0
-                            const bool same = act.par * s.par > 0;
-                            const bool okp = (ct < 2) ? same : !same;
-                            if (!okp) continue;
-                            auto sg = [&](int k) { return seg_point<DEG>(zb, s, k); };
-                            const int ts = s.len - 1;
-                            const int dl = 4 * M;
-                            cd a0, a1, b0, b1;
-                            if (ct == 0) {
-                                tail_line(ag, act.t, dl, a0, a1);
-                                head_line(sg, ts, b0, b1);
-                            } else if (ct == 1) {
-                                head_line(ag, act.t, a0, a1);
-                                tail_line(sg, ts, dl, b0, b1);
-                            } else if (ct == 2) {
-                                head_line(ag, act.t, a0, a1);
-                                head_line(sg, ts, b0, b1);
-                            } else {
-                                tail_line(ag, act.t, dl, a0, a1);
-                                tail_line(sg, ts, dl, b0, b1);
-                            }
-                            if (act.empty) {
-                                a0 = a1 = cd{0.0, 0.0};
-                            }
-                            if (geom_condition(a0, a1, b0, b1)) chosen = u;
+                    }
+                    int chosen = -1;
+                    for (int u = 0; u < nb && chosen < 0; ++u) {
+                        const int ct = bi[u] / R, q = bi[u] % R;
+                        const Seg sq = cand[q];
+                        if (sq.empty) continue;  // zero segments connect only within 1e-5 of 0
+                        const bool same = act.par * sq.par > 0;
+                        const bool okp = (ct < 2) ? same : !same;
+                        if (!okp) continue;
+                        auto sg = [&](int k) { return seg_point<DEG>(zb, sq, k); };
+                        const int ts = sq.len - 1;
+                        const int dl = 4 * M;
+                        cd a0, a1, b0, b1;
+                        if (ct == 0) {
+                            tail_line(ag, act.t, dl, a0, a1);
+                            head_line(sg, ts, b0, b1);
+                        } else if (ct == 1) {
+                            head_line(ag, act.t, a0, a1);
+                            tail_line(sg, ts, dl, b0, b1);
+                        } else if (ct == 2) {
+                            head_line(ag, act.t, a0, a1);
+                            head_line(sg, ts, b0, b1);
+                        } else {
+                            tail_line(ag, act.t, dl, a0, a1);
+                            tail_line(sg, ts, dl, b0, b1);
                         }
-                        if (chosen < 0) continue;
-                        const int ct = bi[chosen] / R, q = bi[chosen] % R;
-                        const Seg s = cand[q];
-                        const Piece fw{s.row, s.start, s.len, 0}, bw{s.row, s.start, s.len, 1};
-                        if (act.np >= kExtMaxPieces) continue;
-                        if (ct == 0) {          // T-H: act + s
-                            act.pc[act.np++] = fw;
-                        } else if (ct == 3) {   // T-T: act + reversed s
-                            act.pc[act.np++] = bw;
-                        } else {                // H-T: s + act;  H-H: reversed s + act
-                            for (int u = act.np; u > 0; --u) act.pc[u] = act.pc[u - 1];
+                        if (act.empty) {
+                            a0 = a1 = cd{0.0, 0.0};
+                        }
+                        if (geom_condition(a0, a1, b0, b1)) chosen = u;
+                    }
+                    if (chosen < 0) continue;
+                    const int ct = bi[chosen] / R, q = bi[chosen] % R;
+                    const Seg sq = cand[q];
+                    const Piece fw{sq.row, sq.start, sq.len, 0}, bw{sq.row, sq.start, sq.len, 1};
+                    if (act.np >= kExtMaxPieces) continue;
+                    const int np0 = act.np, t0 = act.t, par0 = act.par;
+                    __builtin_amdgcn_wave_barrier();
+                    if (ct == 0) {          // T-H: act + s
+                        if (lane == 0) act.pc[np0] = fw;
+                        at = sq.tail;
+                    } else if (ct == 3) {   // T-T: act + reversed s
+                        if (lane == 0) act.pc[np0] = bw;
+                        at = sq.head;
+                    } else {                // H-T: s + act;  H-H: reversed s + act
+                        if (lane == 0) {
+                            for (int u = np0; u > 0; --u) act.pc[u] = act.pc[u - 1];
                             act.pc[0] = (ct == 1) ? fw : bw;
-                            act.np++;
-                            act.par = (ct == 1) ? s.par : -s.par;
                         }
-                        act.t = act.t + (s.len - 1) + 1;
+                        ah = (ct == 1) ? sq.head : sq.tail;
+                    }
+                    if (lane == 0) {
+                        act.np = np0 + 1;
+                        act.t = t0 + (sq.len - 1) + 1;
+                        if (ct == 1 || ct == 2) act.par = (ct == 1) ? sq.par : -sq.par;
                         cand[q].empty = 1;
                     }
-                    if (tid == 0) s_con[ci] = act;
-                    maxm -= 2;
+                    (void)par0;
+                    __builtin_amdgcn_wave_barrier();
                 }
+                if (lane == 0) s_con[ci] = act;
+                __builtin_amdgcn_wave_barrier();
+                maxm -= 2;
             }
-            __syncthreads();
+        }
+        __syncthreads();
     }
     if (tid == 0) {
         C->closed = closed;
-        C->open = closed != (1u << DEG) - 1u;
-        if (C->open) {
+        C->open = all_closed ? 0 : 1;
+        if (!all_closed) {
             for (int ci = 0; ci < 3; ++ci) C->con[ci] = s_con[ci];
         }
     }
