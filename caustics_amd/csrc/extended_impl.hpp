@@ -27,7 +27,8 @@
 
 namespace caus {
 
-constexpr int kExtBlock = 256;
+constexpr int kExtBlock = 256;    // block of the per-source integration kernel
+constexpr int kExtWaveBlock = 64; // block of the per-source bookkeeping kernels (one wave)
 constexpr int kExtMaxSeg = 32;     // >= 3 * (nlenses^2 + 1)
 constexpr int kExtMaxPieces = 48;  // pieces of one stitched contour
 constexpr int kExtSortMax = 4096;  // limb points per source (gaps sorted in LDS)
@@ -217,10 +218,10 @@ __device__ __forceinline__ void permute_pt(cd (&z)[DEG], uint64_t p) {
 }
 
 // _permute_images over points 0..M-1 of buffer b (extended_source.py:34-53)
-template <int DEG>
+template <int DEG, int NT>
 __device__ void chain_images(cd* zb, uint32_t* fb, uint64_t* perm, int M) {
     const int tid = threadIdx.x;
-    for (int k = tid; k < M - 1; k += kExtBlock) {
+    for (int k = tid; k < M - 1; k += NT) {
         cd a[DEG], b[DEG];
         load_pt<DEG>(zb, k, a);
         load_pt<DEG>(zb, k + 1, b);
@@ -228,62 +229,99 @@ __device__ void chain_images(cd* zb, uint32_t* fb, uint64_t* perm, int M) {
     }
     __syncthreads();
     if (tid < 64) {
-        // wave 0: sequential composition, 64 steps per batch, uniform control flow
-        uint64_t pi = pident<DEG>();
+        // wave 0, 64 points per batch: runs of steps without a conflict compose by a
+        // wave-wide scan (composition is associative); a conflict step applies the
+        // greedy rule to the carried rows, then the scan resumes after it.
+        const int lane = tid;
+        const uint64_t id = pident<DEG>();
+        uint64_t pi = id;
         for (int base = 0; base < M; base += 64) {
-            const int k_l = base + tid;
-            const uint64_t v = (k_l < M - 1) ? perm[k_l] : 0ull;
-            uint64_t mine = 0;
+            const int k_l = base + lane;
             const int nb = min(64, M - base);
-            for (int j = 0; j < nb; ++j) {
-                const int k = base + j;
-                const uint64_t nn = __shfl(v, j, 64);
-                if (tid == j) mine = pi;
-                if (k < M - 1) {
-                    if (!(nn & kConflictBit)) {
-                        pi = pcompose<DEG>(nn, pi);
-                    } else {
-                        cd a[DEG], raw[DEG], b[DEG];
-                        load_pt<DEG>(zb, k, raw);
+            const bool valid = k_l < M - 1;
+            const uint64_t e = valid ? perm[k_l] : id;
+            const uint64_t cm = __ballot(valid && (e & kConflictBit));
+            uint64_t mine = 0;
+            int start = 0;
+            while (start < nb) {
+                const uint64_t rest = cm >> start;
+                const int c = rest ? start + __ffsll((long long)rest) - 1 : nb;
+                // x_j = step j for start <= j < c, identity elsewhere; inclusive scan
+                uint64_t x = (lane >= start && lane < c) ? (e & ~kConflictBit) : id;
 #pragma unroll
-                        for (int i = 0; i < DEG; ++i) {
-                            cd t = raw[0];
+                for (int off = 1; off < 64; off <<= 1) {
+                    const uint64_t o = __shfl_up(x, off, 64);
+                    if (lane >= off) x = pcompose<DEG>(x, o);
+                }
+                uint64_t ex = __shfl_up(x, 1, 64);
+                if (lane == 0) ex = id;
+                const int last = (c < nb) ? c : nb - 1;
+                if (lane >= start && lane <= last) mine = pcompose<DEG>(ex, pi);
+                const uint64_t Pc = __shfl(x, (c > start) ? c - 1 : 0, 64);
+                const uint64_t pi_after = (c > start) ? pcompose<DEG>(Pc, pi) : pi;
+                if (c < nb) {
+                    // greedy match_points of the carried rows of point k against point
+                    // k+1, one row per lane: lane i holds a_i, lane j holds b_j
+                    const int k = base + c;
+                    const cd av = (lane < DEG) ? zb[(size_t)k * DEG + pget(pi_after, lane)]
+                                               : cd{0.0, 0.0};
+                    const cd bv = (lane < DEG) ? zb[(size_t)(k + 1) * DEG + lane] : cd{0.0, 0.0};
+                    unsigned used = 0;
+                    uint64_t p = 0;
+                    for (int i = 0; i < DEG; ++i) {
+                        const cd ai = cd{__shfl(av.re, i, 64), __shfl(av.im, i, 64)};
+                        double d = (lane < DEG && !((used >> lane) & 1u)) ? norm2(bv - ai)
+                                                                          : INFINITY;
+                        int bj = lane;
 #pragma unroll
-                            for (int q = 1; q < DEG; ++q) t = (pget(pi, i) == q) ? raw[q] : t;
-                            a[i] = t;
+                        for (int off = 32; off >= 1; off >>= 1) {
+                            const double od = __shfl_xor(d, off, 64);
+                            const int oj = __shfl_xor(bj, off, 64);
+                            if (od < d || (od == d && oj < bj)) {
+                                d = od;
+                                bj = oj;
+                            }
                         }
-                        load_pt<DEG>(zb, k + 1, b);
-                        pi = greedy_match<DEG>(a, b);
+                        used |= 1u << bj;
+                        p |= (uint64_t)bj << (4 * i);
                     }
+                    pi = p;
+                    start = c + 1;
+                } else {
+                    pi = pi_after;
+                    start = nb;
                 }
             }
             if (k_l < M) perm[k_l] = mine;
         }
     }
     __syncthreads();
-    for (int k = tid; k < M; k += kExtBlock) {
+    for (int k = tid; k < M; k += NT) {
+        // gather straight from memory (row i <- raw row p[i]): no register array with a
+        // dynamic index, which the compiler would place in scratch
         const uint64_t p = perm[k];
         cd z[DEG];
-        load_pt<DEG>(zb, k, z);
-        permute_pt<DEG>(z, p);
+#pragma unroll
+        for (int i = 0; i < DEG; ++i) z[i] = zb[(size_t)k * DEG + pget(p, i)];
         store_pt<DEG>(zb, k, z);
         fb[k] = permute_flags<DEG>(fb[k], p);
     }
     __syncthreads();
 }
 
-// block-wide exclusive prefix sum of per-thread counts (kExtBlock threads)
+// block-wide exclusive prefix sum of per-thread counts (NT threads)
+template <int NT>
 __device__ inline int block_exclusive_scan(int v, int* sh, int& total) {
     const int tid = threadIdx.x;
     sh[tid] = v;
     __syncthreads();
-    for (int off = 1; off < kExtBlock; off <<= 1) {
+    for (int off = 1; off < NT; off <<= 1) {
         const int t = (tid >= off) ? sh[tid - off] : 0;
         __syncthreads();
         sh[tid] += t;
         __syncthreads();
     }
-    total = sh[kExtBlock - 1];
+    total = sh[NT - 1];
     const int r = sh[tid] - v;
     __syncthreads();
     return r;
@@ -1212,23 +1250,23 @@ __global__ __launch_bounds__(kExtFlatBlock) void k_ext_limb_init(ExtArgs A) {
 
 // ---- image continuation over the points of buffer `buf` after `round` rounds
 template <int NL>
-__global__ __launch_bounds__(kExtBlock) void k_ext_chain(ExtArgs A, int buf, int round) {
+__global__ __launch_bounds__(kExtWaveBlock) void k_ext_chain(ExtArgs A, int buf, int round) {
     constexpr int DEG = ext_degree<NL>();
     const int64_t s = blockIdx.x;
     const ExtBatch B = ext_batch(A);
     const int M = ext_points(B.npts[s], round);
     ExtWS W = ext_ws(ext_source_base<DEG>(A, s), DEG, A.mmax);
-    chain_images<DEG>(W.z[buf], W.flg[buf], W.perm, M);
+    chain_images<DEG, kExtWaveBlock>(W.z[buf], W.flg[buf], W.perm, M);
 }
 
 // ---- one refinement round, part 1 (extended_source.py:108-135): the n_add largest
 // image gaps (bitonic sort in LDS, descending by (gap, index)), insertion shifts, old
 // points moved to the other buffer, one task per new point
 template <int NL>
-__global__ __launch_bounds__(kExtBlock) void k_ext_select(ExtArgs A, int cur, int round) {
+__global__ __launch_bounds__(kExtWaveBlock) void k_ext_select(ExtArgs A, int cur, int round) {
     constexpr int DEG = ext_degree<NL>();
     extern __shared__ uint64_t s_dyn[];
-    __shared__ int s_int[kExtBlock];
+    __shared__ int s_int[kExtWaveBlock];
     const int tid = threadIdx.x;
     const int64_t src = blockIdx.x;
     const ExtBatch B = ext_batch(A);
@@ -1241,7 +1279,7 @@ __global__ __launch_bounds__(kExtBlock) void k_ext_select(ExtArgs A, int cur, in
     while (P2 < M - 1) P2 <<= 1;
     uint64_t* s_key = s_dyn;
     int* s_idx = reinterpret_cast<int*>(s_dyn + P2);
-    for (int k = tid; k < P2; k += kExtBlock) {
+    for (int k = tid; k < P2; k += kExtWaveBlock) {
         uint64_t key = 0;
         int idx = -1;
         if (k < M - 1) {
@@ -1262,7 +1300,7 @@ __global__ __launch_bounds__(kExtBlock) void k_ext_select(ExtArgs A, int cur, in
     __syncthreads();
     for (int size = 2; size <= P2; size <<= 1) {
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int i = tid; i < P2; i += kExtBlock) {
+            for (int i = tid; i < P2; i += kExtWaveBlock) {
                 const int j = i ^ stride;
                 if (j > i) {
                     const bool desc = (i & size) == 0;
@@ -1282,16 +1320,16 @@ __global__ __launch_bounds__(kExtBlock) void k_ext_select(ExtArgs A, int cur, in
     }
     // selected intervals -> exclusive prefix count = shift of every old point
     int* shift = reinterpret_cast<int*>(W.perm);
-    const int per = (M + kExtBlock - 1) / kExtBlock;
+    const int per = (M + kExtWaveBlock - 1) / kExtWaveBlock;
     const int k0 = tid * per, k1 = min(M, k0 + per);
     for (int k = k0; k < k1; ++k) shift[k] = 0;
     __syncthreads();
-    for (int c = tid; c < nadd; c += kExtBlock) shift[s_idx[c]] = 1;
+    for (int c = tid; c < nadd; c += kExtWaveBlock) shift[s_idx[c]] = 1;
     __syncthreads();
     int cnt = 0;
     for (int k = k0; k < k1; ++k) cnt += shift[k];
     int total;
-    int run = block_exclusive_scan(cnt, s_int, total);
+    int run = block_exclusive_scan<kExtWaveBlock>(cnt, s_int, total);
     for (int k = k0; k < k1; ++k) {
         const int ins = shift[k];
         shift[k] = run;
@@ -1299,7 +1337,7 @@ __global__ __launch_bounds__(kExtBlock) void k_ext_select(ExtArgs A, int cur, in
     }
     __syncthreads();
     const int nxt = cur ^ 1;
-    for (int k = tid; k < M; k += kExtBlock) {
+    for (int k = tid; k < M; k += kExtWaveBlock) {
         const int d = k + shift[k];
         cd z[DEG];
         load_pt<DEG>(zb, k, z);
@@ -1308,7 +1346,7 @@ __global__ __launch_bounds__(kExtBlock) void k_ext_select(ExtArgs A, int cur, in
         W.theta[nxt][d] = W.theta[cur][k];
     }
     const int64_t t0 = B.off_add[src];
-    for (int c = tid; c < nadd; c += kExtBlock) {
+    for (int c = tid; c < nadd; c += kExtWaveBlock) {
         const int k = s_idx[c];
         B.t_src[t0 + c] = (int32_t)src;
         B.t_left[t0 + c] = k;
@@ -1346,9 +1384,13 @@ __global__ __launch_bounds__(kExtFlatBlock) void k_ext_solve_add(ExtArgs A, int 
             B.fail[s] = 1;
         if (A.work) ext_count(A.work, st, 1);
         const uint64_t p = greedy_match<DEG>(left, z);
-        permute_pt<DEG>(z, p);
+        // scatter raw row j to row pinv[j] (output row i = raw row p[i])
+        uint64_t pinv = 0;
+#pragma unroll
+        for (int i = 0; i < DEG; ++i) pinv |= (uint64_t)i << (4 * pget(p, i));
+#pragma unroll
+        for (int j = 0; j < DEG; ++j) W.z[nxt][(size_t)d * DEG + pget(pinv, j)] = z[j];
         f = permute_flags<DEG>(f, p);
-        store_pt<DEG>(W.z[nxt], d, z);
         W.flg[nxt][d] = f;
         W.theta[nxt][d] = th;
     }
@@ -1357,7 +1399,7 @@ __global__ __launch_bounds__(kExtFlatBlock) void k_ext_solve_add(ExtArgs A, int 
 // ---- closed rows, segments and stitching (extended_source.py:156-727) for the final
 // points in buffer `buf`; writes the source's ExtContours
 template <int NL>
-__global__ __launch_bounds__(kExtBlock) void k_ext_contours(ExtArgs A, int buf) {
+__global__ __launch_bounds__(kExtWaveBlock) void k_ext_contours(ExtArgs A, int buf) {
     constexpr int DEG = ext_degree<NL>();
     constexpr int NIMG = NL * NL + 1;
     constexpr int NSEG = 3 * NIMG;  // nr_of_segments
@@ -1381,7 +1423,7 @@ __global__ __launch_bounds__(kExtBlock) void k_ext_contours(ExtArgs A, int buf) 
     if (tid < DEG) s_nst[tid] = s_nen[tid] = 0;
     __syncthreads();
     unsigned am = 0xFFFFu;
-    for (int k = tid; k < M; k += kExtBlock) am &= fb[k];
+    for (int k = tid; k < M; k += kExtWaveBlock) am &= fb[k];
     atomicAnd(&s_allmask, am);
     __syncthreads();
     unsigned closed = 0;
@@ -1397,7 +1439,7 @@ __global__ __launch_bounds__(kExtBlock) void k_ext_contours(ExtArgs A, int buf) 
         // are closed contribute nothing (their segments were zeroed)
         const unsigned open = ((1u << DEG) - 1u) & ~closed;
         uint64_t* cpw = W.perm;
-        for (int c = tid; c <= M; c += kExtBlock) {
+        for (int c = tid; c <= M; c += kExtWaveBlock) {
             uint64_t word = 0;
 #pragma unroll
             for (int r = 0; r < DEG; ++r) {
@@ -1763,21 +1805,22 @@ int ext_run(hipStream_t s, const ExtArgs& a) {
     const int64_t lanes_init = S * (int64_t)((ext_npts_init(a.npts_def) + kExtPer - 1) / kExtPer);
     hipLaunchKernelGGL((k_ext_limb_init<NL, MODE>), dim3(ext_flat_grid(lanes_init)),
                        dim3(kExtFlatBlock), 0, s, a);
-    hipLaunchKernelGGL((k_ext_chain<NL>), dim3(nb), dim3(kExtBlock), 0, s, a, 0, 0);
+    hipLaunchKernelGGL((k_ext_chain<NL>), dim3(nb), dim3(kExtWaveBlock), 0, s, a, 0, 0);
     int P2 = 1;
     while (P2 < a.mmax) P2 <<= 1;
     const size_t lds = (size_t)P2 * (sizeof(uint64_t) + sizeof(int));
     const int add_grid = ext_flat_grid(S * (int64_t)(a.nadd_max > 0 ? a.nadd_max : 1));
     int cur = 0;
     for (int round = 0; round < 10; ++round) {
-        hipLaunchKernelGGL((k_ext_select<NL>), dim3(nb), dim3(kExtBlock), lds, s, a, cur, round);
+        hipLaunchKernelGGL((k_ext_select<NL>), dim3(nb), dim3(kExtWaveBlock), lds, s, a, cur,
+                           round);
         hipLaunchKernelGGL((k_ext_solve_add<NL, MODE>), dim3(add_grid), dim3(kExtFlatBlock), 0, s,
                            a, cur);
         cur ^= 1;
     }
-    hipLaunchKernelGGL((k_ext_chain<NL>), dim3(nb), dim3(kExtBlock), 0, s, a, cur, 10);
+    hipLaunchKernelGGL((k_ext_chain<NL>), dim3(nb), dim3(kExtWaveBlock), 0, s, a, cur, 10);
     if constexpr (NL > 1)
-        hipLaunchKernelGGL((k_ext_contours<NL>), dim3(nb), dim3(kExtBlock), 0, s, a, cur);
+        hipLaunchKernelGGL((k_ext_contours<NL>), dim3(nb), dim3(kExtWaveBlock), 0, s, a, cur);
     hipLaunchKernelGGL((k_ext_integrate<NL, LD, VJP>), dim3(nb), dim3(kExtBlock), 0, s, a, cur);
     return hipGetLastError() == hipSuccess ? CAUS_OK : CAUS_ERR_LAUNCH;
 }
