@@ -53,6 +53,8 @@ __host__ __device__ inline size_t ext_ws_bytes(int deg, int mmax) {
     b += ext_align((size_t)(mmax + 1) * sizeof(uint64_t));        // permutations / changepoints
     b += ext_align(2 * ((size_t)mmax * deg + 2) * sizeof(double)); // P, Q
     b += ext_align(2 * ((size_t)mmax * deg + 2) * sizeof(double)); // point cotangents
+    b += ext_align(2 * (size_t)mmax * sizeof(int32_t));             // limb order x2
+    b += ext_align(2 * (size_t)mmax * sizeof(uint64_t));            // gap keys x2
     return b;
 }
 
@@ -63,6 +65,8 @@ struct ExtWS {
     uint64_t* perm;
     double* pq;
     double* xy;
+    int32_t* ord[2];   // refinement: limb order -> index of the point in append order
+    uint64_t* gk[2];   // refinement: key of the gap after each point in limb order
 };
 
 __device__ inline ExtWS ext_ws(char* base, int deg, int mmax) {
@@ -82,6 +86,12 @@ __device__ inline ExtWS ext_ws(char* base, int deg, int mmax) {
     w.pq = (double*)(base + o);
     o += ext_align(2 * ((size_t)mmax * deg + 2) * sizeof(double));
     w.xy = (double*)(base + o);
+    o += ext_align(2 * ((size_t)mmax * deg + 2) * sizeof(double));
+    w.ord[0] = (int32_t*)(base + o);
+    w.ord[1] = w.ord[0] + mmax;
+    o += ext_align(2 * (size_t)mmax * sizeof(int32_t));
+    w.gk[0] = (uint64_t*)(base + o);
+    w.gk[1] = w.gk[0] + mmax;
     return w;
 }
 
@@ -1020,10 +1030,12 @@ __device__ double contour_ld_fused(const cd* zb, const double* theta, const Piec
 // the arithmetic) run one limb point per lane at the solver's occupancy:
 //   k_ext_plan        1 block      effective npts, prefix offsets of the flat work lists
 //   k_ext_limb_init   flat         initial limb angles, kExtPer consecutive angles per lane
-//   k_ext_chain       per source   image continuation (extended_source.py:34-53)
-//   10 x k_ext_select per source   gap sort, insertion shifts, move old points, new tasks
-//        k_ext_solve_add flat      new points, matched to their left neighbour
-//   k_ext_chain       per source
+//   k_ext_chain       per source   image continuation (extended_source.py:34-53), limb
+//                                  order and gap keys of the initial points
+//   10 x k_ext_select per source   gap-key sort, insertion shifts, next limb order, tasks
+//        k_ext_solve_add flat      new points appended, matched to their left neighbour,
+//                                  keys of their two gaps
+//   k_ext_chain       per source   points gathered in limb order, image continuation
 //   k_ext_contours    per source   closure test, segments, stitching -> piece lists
 //   k_ext_integrate   per source   contour integrals, magnification, reverse mode
 // No launch allocates, synchronises with the host or clears memory with a memset, so
@@ -1073,9 +1085,11 @@ struct ExtBatch {
     int64_t* off_add;   // S + 1: prefix of the new points of one refinement round
     int32_t* npts;      // S: effective limb resolution
     uint8_t* fail;      // S: a solve did not converge, or npts was out of range
-    int32_t* t_src;     // new-point tasks of the current round (capacity S * nadd_max)
-    int32_t* t_left;
-    int32_t* t_dst;
+    int32_t* t_src;     // new-point tasks of the current round (capacity S * nadd_max):
+    int32_t* t_left;    // source, append index of the left and right limb neighbours,
+    int32_t* t_right;   // limb-order position and append index of the new point, and
+    int32_t* t_dst;     // its limb angle
+    int32_t* t_app;
     double* t_th;
 };
 
@@ -1086,7 +1100,7 @@ __host__ __device__ inline size_t ext_task_cap(int64_t S, int nadd_max) {
 __host__ __device__ inline size_t ext_header_bytes(int64_t S, int nadd_max) {
     const size_t cap = ext_task_cap(S, nadd_max);
     return ext_align(2 * (size_t)(S + 1) * sizeof(int64_t)) + ext_align((size_t)S * 4) +
-           ext_align((size_t)S) + ext_align(3 * cap * sizeof(int32_t)) +
+           ext_align((size_t)S) + ext_align(5 * cap * sizeof(int32_t)) +
            ext_align(cap * sizeof(double));
 }
 
@@ -1112,8 +1126,10 @@ __device__ inline ExtBatch ext_batch(const ExtArgs& A) {
     p += ext_align((size_t)A.S);
     b.t_src = (int32_t*)p;
     b.t_left = b.t_src + cap;
-    b.t_dst = b.t_left + cap;
-    p += ext_align(3 * cap * sizeof(int32_t));
+    b.t_right = b.t_left + cap;
+    b.t_dst = b.t_right + cap;
+    b.t_app = b.t_dst + cap;
+    p += ext_align(5 * cap * sizeof(int32_t));
     b.t_th = (double*)p;
     return b;
 }
@@ -1248,20 +1264,58 @@ __global__ __launch_bounds__(kExtFlatBlock) void k_ext_limb_init(ExtArgs A) {
     }
 }
 
-// ---- image continuation over the points of buffer `buf` after `round` rounds
+// key of the gap between limb points a and b (rows as stored): the largest image step
+// |b_r - a_r| over rows real at either end, as an ordered bit pattern
+// (extended_source.py:112-116)
+template <int DEG>
+__device__ __forceinline__ uint64_t gap_key(const cd* za, uint32_t fa, const cd* zb_,
+                                            uint32_t fb_) {
+    const uint32_t m = (fa | fb_) & 0xFFFFu;
+    double dmax = 0.0;
+#pragma unroll
+    for (int r = 0; r < DEG; ++r)
+        if ((m >> r) & 1u) dmax = fmax(dmax, sqrt(norm2(zb_[r] - za[r])));
+    return (uint64_t)__double_as_longlong(dmax);
+}
+
+// ---- image continuation.  final = false: over the initial points (buffer 0, limb
+// order = append order), then the limb order and the gap keys of round 0.  final =
+// true: the refined points are first gathered in limb order (ord[cur]) into buffer 1.
 template <int NL>
-__global__ __launch_bounds__(kExtWaveBlock) void k_ext_chain(ExtArgs A, int buf, int round) {
+__global__ __launch_bounds__(kExtWaveBlock) void k_ext_chain(ExtArgs A, int cur, int final_) {
     constexpr int DEG = ext_degree<NL>();
+    const int tid = threadIdx.x;
     const int64_t s = blockIdx.x;
     const ExtBatch B = ext_batch(A);
-    const int M = ext_points(B.npts[s], round);
     ExtWS W = ext_ws(ext_source_base<DEG>(A, s), DEG, A.mmax);
-    chain_images<DEG, kExtWaveBlock>(W.z[buf], W.flg[buf], W.perm, M);
+    if (!final_) {
+        const int M = ext_points(B.npts[s], 0);
+        chain_images<DEG, kExtWaveBlock>(W.z[0], W.flg[0], W.perm, M);
+        for (int k = tid; k < M; k += kExtWaveBlock) {
+            W.ord[0][k] = k;
+            if (k + 1 < M)
+                W.gk[0][k] = gap_key<DEG>(W.z[0] + (size_t)k * DEG, W.flg[0][k],
+                                          W.z[0] + (size_t)(k + 1) * DEG, W.flg[0][k + 1]);
+        }
+    } else {
+        const int M = ext_points(B.npts[s], 10);
+        const int32_t* ord = W.ord[cur];
+        for (int k = tid; k < M; k += kExtWaveBlock) {
+            const int a = ord[k];
+#pragma unroll
+            for (int r = 0; r < DEG; ++r) W.z[1][(size_t)k * DEG + r] = W.z[0][(size_t)a * DEG + r];
+            W.flg[1][k] = W.flg[0][a];
+            W.theta[1][k] = W.theta[0][a];
+        }
+        __syncthreads();
+        chain_images<DEG, kExtWaveBlock>(W.z[1], W.flg[1], W.perm, M);
+    }
 }
 
 // ---- one refinement round, part 1 (extended_source.py:108-135): the n_add largest
-// image gaps (bitonic sort in LDS, descending by (gap, index)), insertion shifts, old
-// points moved to the other buffer, one task per new point
+// image gaps (bitonic sort of the gap keys in LDS, descending by (gap, position)),
+// insertion shifts, the next limb order and the carried-over gap keys, one task per new
+// point.  Points stay where they are (append order); only the order and keys move.
 template <int NL>
 __global__ __launch_bounds__(kExtWaveBlock) void k_ext_select(ExtArgs A, int cur, int round) {
     constexpr int DEG = ext_degree<NL>();
@@ -1273,29 +1327,16 @@ __global__ __launch_bounds__(kExtWaveBlock) void k_ext_select(ExtArgs A, int cur
     const int n = B.npts[src];
     const int M = ext_points(n, round), nadd = ext_n_add(n);
     ExtWS W = ext_ws(ext_source_base<DEG>(A, src), DEG, A.mmax);
-    const cd* zb = W.z[cur];
-    const uint32_t* fb = W.flg[cur];
+    const int nxt = cur ^ 1;
+    const int32_t* ord = W.ord[cur];
+    const uint64_t* gk = W.gk[cur];
     int P2 = 1;
     while (P2 < M - 1) P2 <<= 1;
     uint64_t* s_key = s_dyn;
     int* s_idx = reinterpret_cast<int*>(s_dyn + P2);
     for (int k = tid; k < P2; k += kExtWaveBlock) {
-        uint64_t key = 0;
-        int idx = -1;
-        if (k < M - 1) {
-            cd a[DEG], b[DEG];
-            load_pt<DEG>(zb, k, a);
-            load_pt<DEG>(zb, k + 1, b);
-            const uint32_t m = (fb[k] | fb[k + 1]) & 0xFFFFu;
-            double dmax = 0.0;
-#pragma unroll
-            for (int r = 0; r < DEG; ++r)
-                if ((m >> r) & 1u) dmax = fmax(dmax, sqrt(norm2(b[r] - a[r])));
-            key = (uint64_t)__double_as_longlong(dmax);
-            idx = k;
-        }
-        s_key[k] = key;
-        s_idx[k] = idx;
+        s_key[k] = (k < M - 1) ? gk[k] : 0ull;
+        s_idx[k] = (k < M - 1) ? k : -1;
     }
     __syncthreads();
     for (int size = 2; size <= P2; size <<= 1) {
@@ -1318,7 +1359,7 @@ __global__ __launch_bounds__(kExtWaveBlock) void k_ext_select(ExtArgs A, int cur
             __syncthreads();
         }
     }
-    // selected intervals -> exclusive prefix count = shift of every old point
+    // selected gaps -> exclusive prefix count = shift of every point
     int* shift = reinterpret_cast<int*>(W.perm);
     const int per = (M + kExtWaveBlock - 1) / kExtWaveBlock;
     const int k0 = tid * per, k1 = min(M, k0 + per);
@@ -1332,31 +1373,34 @@ __global__ __launch_bounds__(kExtWaveBlock) void k_ext_select(ExtArgs A, int cur
     int run = block_exclusive_scan<kExtWaveBlock>(cnt, s_int, total);
     for (int k = k0; k < k1; ++k) {
         const int ins = shift[k];
-        shift[k] = run;
+        shift[k] = run | (ins << 30);   // bit 30: the gap after point k is refined
         run += ins;
     }
     __syncthreads();
-    const int nxt = cur ^ 1;
     for (int k = tid; k < M; k += kExtWaveBlock) {
-        const int d = k + shift[k];
-        cd z[DEG];
-        load_pt<DEG>(zb, k, z);
-        store_pt<DEG>(W.z[nxt], d, z);
-        W.flg[nxt][d] = fb[k];
-        W.theta[nxt][d] = W.theta[cur][k];
+        const int sh = shift[k];
+        const int d = k + (sh & 0x3FFFFFFF);
+        W.ord[nxt][d] = ord[k];
+        if (k + 1 < M && !(sh >> 30)) W.gk[nxt][d] = gk[k];
     }
     const int64_t t0 = B.off_add[src];
     for (int c = tid; c < nadd; c += kExtWaveBlock) {
         const int k = s_idx[c];
+        const int d = k + (shift[k] & 0x3FFFFFFF) + 1;
+        const int a = ord[k], b = ord[k + 1];
         B.t_src[t0 + c] = (int32_t)src;
-        B.t_left[t0 + c] = k;
-        B.t_dst[t0 + c] = k + shift[k] + 1;
-        B.t_th[t0 + c] = 0.5 * (W.theta[cur][k] + W.theta[cur][k + 1]);
+        B.t_left[t0 + c] = a;
+        B.t_right[t0 + c] = b;
+        B.t_dst[t0 + c] = d;
+        B.t_app[t0 + c] = M + c;
+        B.t_th[t0 + c] = 0.5 * (W.theta[0][a] + W.theta[0][b]);
+        W.ord[nxt][d] = M + c;
     }
 }
 
 // ---- one refinement round, part 2: every new point of the batch, one per lane,
-// solved and matched to its left neighbour's images (match_points)
+// solved, matched to its left neighbour's images (match_points), appended, and the
+// keys of its two gaps
 template <int NL, int MODE>
 __global__ __launch_bounds__(kExtFlatBlock) void k_ext_solve_add(ExtArgs A, int cur) {
     constexpr int DEG = ext_degree<NL>();
@@ -1366,7 +1410,7 @@ __global__ __launch_bounds__(kExtFlatBlock) void k_ext_solve_add(ExtArgs A, int 
     const int nxt = cur ^ 1;
     for (int64_t t = (int64_t)blockIdx.x * kExtFlatBlock + threadIdx.x; t < total; t += stride) {
         const int64_t s = B.t_src[t];
-        const int k = B.t_left[t], d = B.t_dst[t];
+        const int a = B.t_left[t], b = B.t_right[t], d = B.t_dst[t], e = B.t_app[t];
         const double th = B.t_th[t];
         LensFrame<NL> lf;
         cd w0;
@@ -1376,7 +1420,7 @@ __global__ __launch_bounds__(kExtFlatBlock) void k_ext_solve_add(ExtArgs A, int 
         double sn, cs;
         sincos(th, &sn, &cs);
         cd z[DEG], left[DEG];
-        load_pt<DEG>(W.z[cur], k, left);
+        load_pt<DEG>(W.z[0], a, left);
         uint32_t f;
         SolveStats st{0, 0, 0, 0};
         if (!ext_solve_point<NL, MODE>(w0 + rho * cd{cs, sn}, lf, z, f, A.itmax,
@@ -1388,11 +1432,20 @@ __global__ __launch_bounds__(kExtFlatBlock) void k_ext_solve_add(ExtArgs A, int 
         uint64_t pinv = 0;
 #pragma unroll
         for (int i = 0; i < DEG; ++i) pinv |= (uint64_t)i << (4 * pget(p, i));
+        cd* ze = W.z[0] + (size_t)e * DEG;
 #pragma unroll
-        for (int j = 0; j < DEG; ++j) W.z[nxt][(size_t)d * DEG + pget(pinv, j)] = z[j];
+        for (int j = 0; j < DEG; ++j) ze[pget(pinv, j)] = z[j];
         f = permute_flags<DEG>(f, p);
-        W.flg[nxt][d] = f;
-        W.theta[nxt][d] = th;
+        W.flg[0][e] = f;
+        W.theta[0][e] = th;
+        // gaps (left, new) at d-1 and (new, right) at d, from the stored rows
+        cd zn[DEG];
+#pragma unroll
+        for (int i = 0; i < DEG; ++i) zn[i] = z[pget(p, i)];
+        W.gk[nxt][d - 1] = gap_key<DEG>(left, W.flg[0][a], zn, f);
+        cd right[DEG];
+        load_pt<DEG>(W.z[0], b, right);
+        W.gk[nxt][d] = gap_key<DEG>(zn, f, right, W.flg[0][b]);
     }
 }
 
@@ -1805,7 +1858,7 @@ int ext_run(hipStream_t s, const ExtArgs& a) {
     const int64_t lanes_init = S * (int64_t)((ext_npts_init(a.npts_def) + kExtPer - 1) / kExtPer);
     hipLaunchKernelGGL((k_ext_limb_init<NL, MODE>), dim3(ext_flat_grid(lanes_init)),
                        dim3(kExtFlatBlock), 0, s, a);
-    hipLaunchKernelGGL((k_ext_chain<NL>), dim3(nb), dim3(kExtWaveBlock), 0, s, a, 0, 0);
+    hipLaunchKernelGGL((k_ext_chain<NL>), dim3(nb), dim3(kExtWaveBlock), 0, s, a, 0, 0);  // initial
     int P2 = 1;
     while (P2 < a.mmax) P2 <<= 1;
     const size_t lds = (size_t)P2 * (sizeof(uint64_t) + sizeof(int));
@@ -1818,10 +1871,10 @@ int ext_run(hipStream_t s, const ExtArgs& a) {
                            a, cur);
         cur ^= 1;
     }
-    hipLaunchKernelGGL((k_ext_chain<NL>), dim3(nb), dim3(kExtWaveBlock), 0, s, a, cur, 10);
+    hipLaunchKernelGGL((k_ext_chain<NL>), dim3(nb), dim3(kExtWaveBlock), 0, s, a, cur, 1);  // final
     if constexpr (NL > 1)
-        hipLaunchKernelGGL((k_ext_contours<NL>), dim3(nb), dim3(kExtWaveBlock), 0, s, a, cur);
-    hipLaunchKernelGGL((k_ext_integrate<NL, LD, VJP>), dim3(nb), dim3(kExtBlock), 0, s, a, cur);
+        hipLaunchKernelGGL((k_ext_contours<NL>), dim3(nb), dim3(kExtWaveBlock), 0, s, a, 1);
+    hipLaunchKernelGGL((k_ext_integrate<NL, LD, VJP>), dim3(nb), dim3(kExtBlock), 0, s, a, 1);
     return hipGetLastError() == hipSuccess ? CAUS_OK : CAUS_ERR_LAUNCH;
 }
 
