@@ -418,14 +418,35 @@ __device__ inline void gauss_legendre(int n, double* x, double* w) {
     }
 }
 
+// sqrt(x) for x > 0 in the normal range (x >= 2^-767), with inv ~ 1/sqrt(x): the
+// v_rsq_f64 seed, one Newton step on h ~ 1/(2 sqrt x) and two residual corrections of
+// g ~ sqrt x (the same sequence as the compiler's sqrt, which adds range scaling and
+// special cases that these callers do not need), then one more step on h for inv.
+__device__ __forceinline__ double sqrt_rsq(double x, double& inv) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double e = fma(-h, g, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    g = fma(d, h, g);
+    inv = 2.0 * fma(h, fma(-h, g, 0.5), h);
+    return g;
+}
+
 // Brightness profile at z (integrate.py:29-44): r = |lens_eq(z) - w0| / rho,
-// B = 3/(3-u1) (u1 B_r + 1 - 2 u1).  Contraction is off here so that the forward-only
-// and the forward+VJP kernels, which inline this into different code, round B alike.
+// B = c3 (u1 B_r + 1 - 2 u1) with c3 = 3/(3-u1), B_r = 1 + sqrt(1 - r^2) (r <= 1) or
+// 1 - sqrt(1 - 1/r^2).  Also returned for the adjoint: v_j = 1/(conj z - conj r_j),
+// D = lens_eq(z) - w0, 1/|D| and dB_r/dr.  Contraction is off here so that the
+// forward-only and the forward+VJP kernels, which inline this into different code, round
+// B alike.
 template <int NL>
 __device__ __forceinline__ double ld_brightness_core(cd z, const LensFrame<NL>& lf, cd w0,
-                                                     double irho, double u1, cd (&v)[NL], cd& D,
-                                                     double& aD, double& r, double& Br,
-                                                     double& c3) {
+                                                     double irho, double u1, double c3,
+                                                     cd (&v)[NL], cd& D, double& iaD,
+                                                     double& r, double& Br, double& dBr) {
 #pragma clang fp contract(off)
     const cd zb = conj(z);
     cd G = cd{0.0, 0.0};
@@ -435,24 +456,35 @@ __device__ __forceinline__ double ld_brightness_core(cd z, const LensFrame<NL>& 
         G = cd{fma(lf.e[j], v[j].re, G.re), fma(lf.e[j], v[j].im, G.im)};  // explicit: no
     }                                                                       // contraction choice
     D = z - G - w0;
-    aD = sqrt(norm2(D));
-    r = aD * irho;
-    if (r <= 1.0) {
-        Br = 1.0 + sqrt(fmax(1.0 - r * r, 0.0));
-    } else {
-        const double ir = rcp(r);
-        Br = 1.0 - sqrt(fmax(1.0 - ir * ir, 0.0));
-    }
-    c3 = 3.0 / (3.0 - u1);
+    const double n2 = norm2(D);
+    double ia;
+    const double aD = sqrt_rsq(n2, ia);
+    const bool pos = n2 > 0.0;
+    iaD = pos ? ia : 0.0;
+    r = pos ? aD * irho : 0.0;
+    const bool inside = r <= 1.0;
+    double q = r;
+    if (!inside) q = rcp(r);
+    const double x = 1.0 - q * q;
+    double is;
+    const double sq = sqrt_rsq(x, is);
+    const bool xp = x > 0.0;
+    const double s = xp ? sq : 0.0;
+    is = xp ? is : 0.0;
+    Br = inside ? 1.0 + s : 1.0 - s;
+    dBr = inside ? -r * is : -(q * q * q) * is;
     return c3 * (u1 * Br + 1.0 - 2.0 * u1);
 }
 
+// c3 = 3/(3-u1), as the brightness uses it
+__device__ __forceinline__ double ld_c3(double u1) { return 3.0 / (3.0 - u1); }
+
 template <int NL>
 __device__ __forceinline__ double ld_brightness(cd z, const LensFrame<NL>& lf, cd w0, double irho,
-                                                double u1) {
+                                                double u1, double c3) {
     cd v[NL], D;
-    double aD, r, Br, c3;
-    return ld_brightness_core<NL>(z, lf, w0, irho, u1, v, D, aD, r, Br, c3);
+    double iaD, r, Br, dBr;
+    return ld_brightness_core<NL>(z, lf, w0, irho, u1, c3, v, D, iaD, r, Br, dBr);
 }
 
 // one trapezoid term of the limb-darkened contour integral (integrate.py:118-121),
@@ -478,6 +510,7 @@ __device__ __forceinline__ double ld_line(bool isP, double fixed, double a, doub
     // explicit fma: the forward+VJP twin (ld_line_adj) must round the split point alike
     double sp = (b > a) ? fma(-2.0, rho, b) : fma(2.0, rho, b);
     if (0.5 * d <= 2.0 * rho) sp = fma(0.5, d, a);
+    const double c3 = ld_c3(u1);
     double I = 0.0;
     for (int part = 0; part < 2; ++part) {
         const double lo = part == 0 ? a : sp, hi = part == 0 ? sp : b;
@@ -490,7 +523,7 @@ __device__ __forceinline__ double ld_line(bool isP, double fixed, double a, doub
         for (int i = 0; i < n; ++i) {
             const double v = fma(h, x[i], c);
             const cd zz = isP ? cd{fixed, v} : cd{v, fixed};
-            s = fma(h * ld_brightness<NL>(zz, lf, w0, irho, u1), wt[i], s);
+            s = fma(h * ld_brightness<NL>(zz, lf, w0, irho, u1, c3), wt[i], s);
         }
         I += s;
     }
@@ -672,8 +705,9 @@ __device__ inline cd contour_locate(const cd* zb, const Piece* pc, int np, int k
 // Reverse-mode accumulators of the direct (not through the contour points) dependence
 // of the limb-darkened line integrals on the lens frame, the source centre, rho and u1.
 // A direction with tangents (de_j, dr_j, dw0, drho, du1) receives
-//   sum_j de_j ge_j + Re(gr_j conj(dr_j)) + Re(gw conj(dw0)) + grho drho + gu1 du1,
-// so that each Gauss-Legendre node costs O(NL) instead of O(NL x directions).
+//   sum_j de_j ge_j + e_j Re(gr_j conj(dr_j)) + Re(gw conj(dw0)) + grho drho + gu1 du1
+// (the mass e_j of the dr_j term is applied once, in along), so that each
+// Gauss-Legendre node costs O(NL) instead of O(NL x directions).
 template <int NL>
 struct LDAdj {
     double ge[NL];
@@ -699,11 +733,12 @@ struct LDAdj {
         grho += s * o.grho;
         gu1 += s * o.gu1;
     }
-    __device__ double along(const LensTangent<NL>& lt, cd dw0, double drho, double du1) const {
+    __device__ double along(const LensTangent<NL>& lt, const double (&e)[NL], cd dw0, double drho,
+                            double du1) const {
         double v = grho * drho + gu1 * du1 + gw.re * dw0.re + gw.im * dw0.im;
 #pragma unroll
         for (int j = 0; j < NL; ++j)
-            v += lt.de[j] * ge[j] + gr[j].re * lt.dr[j].re + gr[j].im * lt.dr[j].im;
+            v += lt.de[j] * ge[j] + e[j] * (gr[j].re * lt.dr[j].re + gr[j].im * lt.dr[j].im);
         return v;
     }
 };
@@ -712,41 +747,39 @@ struct LDAdj {
 // parameters (Bx, By); c * (direct parameter derivative of B) is added to A.
 // r = |D|/rho with D = lens_eq(z) - w0;  dr = Re(conj(D) dD)/(|D| rho) - r drho/rho,
 // dD = -(dG + dw0),  dG = sum_j de_j v_j + e_j v_j^2 conj(dr_j),  v_j = 1/(conj z - conj r_j).
+// dc3 = dc3/du1 = c3^2/3.
 template <int NL>
-__device__ inline double ld_brightness_adj(cd z, const LensFrame<NL>& lf, cd w0, double rho,
-                                           double irho, double u1, double c, double& Bx,
+__device__ inline double ld_brightness_adj(cd z, const LensFrame<NL>& lf, cd w0, double irho,
+                                           double u1, double c3, double dc3, double c, double& Bx,
                                            double& By, LDAdj<NL>& A) {
     cd v[NL], D;
-    double aD, r, Br, c3;
-    const double B = ld_brightness_core<NL>(z, lf, w0, irho, u1, v, D, aD, r, Br, c3);
-    cd K = cd{0.0, 0.0};
+    double iaD, r, Br, dBr;
+    const double B = ld_brightness_core<NL>(z, lf, w0, irho, u1, c3, v, D, iaD, r, Br, dBr);
+    cd vv[NL];
 #pragma unroll
-    for (int j = 0; j < NL; ++j) K = K + lf.e[j] * (v[j] * v[j]);
-    double dBr;
-    if (r <= 1.0) {
-        const double x = 1.0 - r * r;
-        dBr = x > 0.0 ? -r / sqrt(x) : 0.0;
-    } else {
-        const double ir = rcp(r);
-        const double x = 1.0 - ir * ir;
-        dBr = x > 0.0 ? -(ir * ir * ir) / sqrt(x) : 0.0;
-    }
+    for (int j = 0; j < NL; ++j) vv[j] = v[j] * v[j];
+    cd K = lf.e[0] * vv[0];
+#pragma unroll
+    for (int j = 1; j < NL; ++j) K = K + lf.e[j] * vv[j];
     const double dBdr = c3 * u1 * dBr;
-    const double iDr = (aD > 0.0) ? rcp(aD * rho) : 0.0;
-    const cd dDx = cd{1.0, 0.0} + K;
-    const cd dDy = cd{0.0, 1.0} * (cd{1.0, 0.0} - K);
-    Bx = dBdr * (D.re * dDx.re + D.im * dDx.im) * iDr;
-    By = dBdr * (D.re * dDy.re + D.im * dDy.im) * iDr;
-    const double dBdu1 = c3 * c3 / 3.0 * (u1 * Br + 1.0 - 2.0 * u1) + c3 * (Br - 2.0);
-    const double a = c * dBdr * iDr;
+    const double iDr = iaD * irho;
+    // dD/dx = 1 + K, dD/dy = i (1 - K)
+    const double gx = D.re * (1.0 + K.re) + D.im * K.im;
+    const double gy = D.re * K.im + D.im * (1.0 - K.re);
+    const double f = dBdr * iDr;
+    Bx = f * gx;
+    By = f * gy;
+    const double dBdu1 = dc3 * (u1 * Br + 1.0 - 2.0 * u1) + c3 * (Br - 2.0);
+    const double t = c * dBdr * irho;
+    const double a = t * iaD;
     const cd Dc = conj(D);
     A.gw = A.gw - a * D;
 #pragma unroll
     for (int j = 0; j < NL; ++j) {
         A.ge[j] -= a * (D.re * v[j].re + D.im * v[j].im);
-        A.gr[j] = A.gr[j] - (a * lf.e[j]) * (Dc * (v[j] * v[j]));
+        A.gr[j] = A.gr[j] - a * (Dc * vv[j]);
     }
-    A.grho -= c * dBdr * r * irho;
+    A.grho -= t * r;
     A.gu1 += c * dBdu1;
     return B;
 }
@@ -779,6 +812,7 @@ __device__ inline double ld_line_adj(bool isP, double fixed, double a, double b,
         dsp_db = 1.0;
         dsp_drho = 2.0;
     }
+    const double c3 = ld_c3(u1), dc3 = c3 * c3 / 3.0;
     double S = 0.0, dS_lo[2], dS_hi[2];
     dfix = 0.0;
     for (int part = 0; part < 2; ++part) {
@@ -788,23 +822,32 @@ __device__ inline double ld_line_adj(bool isP, double fixed, double a, double b,
         const int n = part == 0 ? g.n1 : g.n2;
         const double h = 0.5 * (hi - lo), c = 0.5 * (hi + lo);
         const double cw = wA * sc * h;
-        double s = 0.0, slo = 0.0, shi = 0.0;
+        // s: the forward sum, rounded as ld_line rounds it; S0 = sum w B,
+        // S1 = sum w dB/dt, S2 = sum w x dB/dt, S3 = sum w dB/d(fixed) give the
+        // derivatives along lo, hi and the fixed coordinate
+        double s = 0.0, S0 = 0.0, S1 = 0.0, S2 = 0.0, S3 = 0.0;
 #pragma unroll 1
         for (int i = 0; i < n; ++i) {
-            const double t = fma(h, x[i], c);
+            const double xi = x[i], wi = wt[i];
+            const double t = fma(h, xi, c);
             const cd zz = isP ? cd{fixed, t} : cd{t, fixed};
             double Bx, By;
-            const double B = ld_brightness_adj<NL>(zz, lf, w0, rho, irho, u1, cw * wt[i], Bx, By, A);
+            const double B = ld_brightness_adj<NL>(zz, lf, w0, irho, u1, c3, dc3, cw * wi, Bx, By,
+                                                   A);
             const double fl = isP ? By : Bx;   // along the line
             const double ff = isP ? Bx : By;   // across (fixed coordinate)
-            s = fma(h * B, wt[i], s);
-            slo += wt[i] * (-0.5 * B + 0.5 * h * fl * (1.0 - x[i]));
-            shi += wt[i] * (0.5 * B + 0.5 * h * fl * (1.0 + x[i]));
-            dfix += h * wt[i] * ff;
+            s = fma(h * B, wi, s);
+            S0 = fma(wi, B, S0);
+            const double wfl = wi * fl;
+            S1 += wfl;
+            S2 = fma(wfl, xi, S2);
+            S3 = fma(wi, ff, S3);
         }
         S += s;
-        dS_lo[part] = slo;
-        dS_hi[part] = shi;
+        // d/dlo of h sum w B(c + h x) with dh/dlo = -1/2, dc/dlo = 1/2, and d/dhi
+        dS_lo[part] = fma(0.5 * h, S1 - S2, -0.5 * S0);
+        dS_hi[part] = fma(0.5 * h, S1 + S2, 0.5 * S0);
+        dfix = fma(h, S3, dfix);
     }
     const double dsp = dS_hi[0] + dS_lo[1];
     da = dS_lo[0] + dsp * dsp_da;
@@ -837,7 +880,7 @@ __device__ inline void image_adjoint(cd z, double th, const LensFrame<NL>& lf, d
         const cd v = it.inv[j];
         const cd cv = c * v;
         A.ge[j] += cv.re;
-        A.gr[j] = A.gr[j] + lf.e[j] * (cv * v);
+        A.gr[j] = A.gr[j] + cv * v;
     }
 }
 
@@ -1042,6 +1085,12 @@ __device__ double contour_ld_fused(const cd* zb, const double* theta, const Piec
 // the sequence can be captured in a HIP graph.
 // ============================================================================
 namespace caus {
+
+// waves per SIMD that the register allocation of the reverse-mode integrate kernel
+// targets (its per-thread adjoint state is large)
+#ifndef CAUS_EXT_VJP_WAVES
+#define CAUS_EXT_VJP_WAVES 2
+#endif
 
 constexpr int kExtPer = 4;          // initial limb angles per lane of k_ext_limb_init
 constexpr int kExtFlatBlock = 256;  // block of the flat kernels
@@ -1747,7 +1796,8 @@ __global__ __launch_bounds__(kExtWaveBlock) void k_ext_contours(ExtArgs A, int b
 // ---- contour integrals, magnification and its reverse mode (extended_source.py:
 // 860-901; integrate.py:18-121)
 template <int NL, bool LD, bool VJP>
-__global__ __launch_bounds__(kExtBlock) __attribute__((amdgpu_waves_per_eu(VJP ? 2 : 3)))
+__global__ __launch_bounds__(kExtBlock)
+__attribute__((amdgpu_waves_per_eu(VJP ? CAUS_EXT_VJP_WAVES : 3)))
 void k_ext_integrate(ExtArgs A, int buf) {
     constexpr int DEG = ext_degree<NL>();
     constexpr int ND = ext_nd<NL>();
@@ -1832,7 +1882,8 @@ void k_ext_integrate(ExtArgs A, int buf) {
         ExtTangents<NL> T;
         ext_tangents<NL>(A.params + src * A.pstride, T);
 #pragma unroll
-        for (int d = 0; d < ND; ++d) gacc[d] = adj.along(T.lt[d], T.dw0[d], T.drho[d], T.du1[d]);
+        for (int d = 0; d < ND; ++d)
+            gacc[d] = adj.along(T.lt[d], lf.e, T.dw0[d], T.drho[d], T.du1[d]);
 #pragma unroll
         for (int d = 0; d < ND; ++d) {
             double v = block_sum(gacc[d], s_d) * base;
