@@ -145,8 +145,8 @@ int caus_hexadecapole_vjp(void* stream, int32_t nlenses, const double* w, int64_
                           int64_t rho_stride, double u1, const double* g_mu, double* mu_hex,
                           double* g_out, uint8_t* status, int32_t itmax);
 
-/* Extended-source magnification by contour integration, one workgroup per source
- * (reference: mag_extended_source, src/caustics/extended_source.py:730-901, with
+/* Extended-source magnification by contour integration for a batch of sources, as one
+ * fixed sequence of launches over the batch (reference: mag_extended_source, src/caustics/extended_source.py:730-901, with
  * _images_of_source_limb :56-153, segments and stitching :156-727 and the integrals of
  * src/caustics/integrate.py:18-121).
  *   w0 (S,) complex source centres; rho (S,); params rows as for caus_mag_point_source
