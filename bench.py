@@ -60,7 +60,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--roots-only", action="store_true")
-    ap.add_argument("--workloads", default="c1,c3,c4,c5",
+    ap.add_argument("--workloads", default="c1,c3,c4,c5,ll",
                     help="comma-separated extra workloads to run after the root solve")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a kernel of "
@@ -171,7 +171,8 @@ def main():
                 ("c1", "point_source_1e4", lambda: bench_point_source(args.steps, 2)),
                 ("c3", "lightcurves_256x2000_fwd_vjp", lambda: bench_lightcurves(3, 1)),
                 ("c4", "extended_binary_ld_4096_fwd_vjp", lambda: bench_extended_binary(3, 1)),
-                ("c5", "triple_extended_lightcurves_fwd_vjp", lambda: bench_triple(1, 0))):
+                ("c5", "triple_extended_lightcurves_fwd_vjp", lambda: bench_triple(1, 0)),
+                ("ll", "marginalized_log_likelihood_grad", lambda: bench_likelihood(20, 3))):
             if key not in sel:
                 continue
             print(f"bench: {name}", file=sys.stderr, flush=True)
@@ -433,6 +434,68 @@ def bench_triple(steps, warmup, n_sets=1024, n_epochs=1500, npts=256, budget_s=4
     out["cf32"].update({"max_rel_err_vs_fp64": float(e.max()),
                         "p99_rel_err_vs_fp64": float(np.quantile(e, 0.99)),
                         "frac_le_1e-8": float(np.mean(e <= 1e-8))})
+    return out
+
+
+def bench_likelihood(steps, warmup, n_epochs=1500, lens=(900, 600)):
+    """Flux-marginalised log-likelihood with its gradient w.r.t. the magnifications
+    (src/caustics/linalg.py:56-70) for B magnification curves over 1500 epochs of two
+    observatories: B = 1024 (one HMC step of the C5 parameter sets) and B = 16384 (large
+    enough to price the kernel against HBM).  Algorithmic bytes per launch: A read once
+    and g_A written once, 16 B per curve-epoch (fobs and C_inv, shared by all curves,
+    add 16 B per epoch); HIP events on the stream the kernel is launched on."""
+    import torch
+
+    from caustics_amd import _lib
+
+    g = np.random.default_rng(11)
+    n = n_epochs
+    t_host = np.sort(g.uniform(-40, 40, n))
+    t = torch.as_tensor(t_host, device="cuda")
+    f_host = 800.0 * (1.0 + 1.0 / (1.0 + (t_host / 10.0) ** 2)) + 200.0
+    fobs = torch.as_tensor(f_host + g.normal(0, 10.0, n), device="cuda")
+    c_inv = torch.as_tensor(1.0 / (10.0 * g.uniform(0.7, 1.3, n)) ** 2, device="cuda")
+    off = torch.as_tensor(np.concatenate([[0], np.cumsum(lens)]), dtype=torch.int64,
+                          device="cuda")
+    L = _lib.lib()
+    out = {}
+    for B in (1024, 16384):
+        u0 = torch.as_tensor(g.uniform(0.05, 0.6, B), device="cuda")[:, None]
+        tE = torch.as_tensor(g.uniform(8, 40, B), device="cuda")[:, None]
+        u = torch.hypot(u0, t[None, :] / tE)
+        A = ((u * u + 2) / (u * torch.sqrt(u * u + 4))).contiguous()
+        ll = torch.empty(B, dtype=torch.float64, device="cuda")
+        beta = torch.empty((B, 2, 2), dtype=torch.float64, device="cuda")
+        gA = torch.empty((B, n), dtype=torch.float64, device="cuda")
+        stream = torch.cuda.current_stream()
+
+        def call():
+            rc = L.caus_marginalized_log_likelihood(
+                _lib.stream_ptr(), _lib.ptr(A), B, n, n, _lib.ptr(fobs), _lib.ptr(c_inv),
+                _lib.ptr(off), len(lens), _lib.ptr(ll), _lib.ptr(beta), None, _lib.ptr(gA), n)
+            assert rc == 0
+
+        for _ in range(warmup):
+            call()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            call()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        nbytes = 16.0 * B * n + 16.0 * n
+        out[f"B{B}"] = {"curves": B, "epochs": n, "ms": ms, "curves_per_s": B / ms * 1e3,
+                        "roofline": {"bound": "hbm", "achieved_GBps": nbytes / ms / 1e6,
+                                     "peak_GBps": 8000.0,
+                                     "frac": nbytes / ms / 1e6 / 8000.0,
+                                     "bytes_per_launch": nbytes},
+                        "finite": bool(torch.isfinite(ll).all().item())}
+        print(f"bench: likelihood B={B}: {ms * 1e3:.1f} us/launch, "
+              f"{nbytes / ms / 1e6:.0f} GB/s", file=sys.stderr, flush=True)
+    out["workload"] = ("flux-marginalised log-likelihood + d ll/dA, 1500 epochs in two "
+                       "observatories (900 + 600), diagonal covariance")
     return out
 
 

@@ -8,6 +8,7 @@ from .ehrlich_aberth_primitive import ehrlich_aberth, poly_roots
 from .extended_source import mag_extended_source
 from .lightcurve import (hexadecapole_gate, hexadecapole_vjp, mag, mag_fwd_vjp,
                          mag_point_source_lightcurve)
+from .linalg import marginalized_log_likelihood
 from .point_source import critical_and_caustic_curves, lens_eq, lens_eq_det_jac, mag_point_source
 
 __all__ = [
@@ -23,4 +24,5 @@ __all__ = [
     "hexadecapole_vjp",
     "mag",
     "mag_fwd_vjp",
+    "marginalized_log_likelihood",
 ]

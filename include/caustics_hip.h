@@ -205,6 +205,25 @@ int caus_mag_extended_source_work(void* stream, int32_t nlenses, const double* w
 int caus_match_chain(void* stream, const double* pts, double* out, int64_t npts, int32_t n,
                      int64_t nchains);
 
+/* Flux-marginalised log-likelihood with a diagonal data covariance for B magnification
+ * curves that share one set of observations (reference: marginalized_log_likelihood,
+ * src/caustics/linalg.py:10-70, dense_covariance=False; the lists of the reference are
+ * the nseg segments here).
+ *   A (B, n) magnifications, row stride lda >= n; the n epochs are nseg observatory
+ *     segments, epochs seg_off[k] .. seg_off[k+1]-1 (seg_off (nseg+1,) int64 on the
+ *     device, seg_off[0] = 0, seg_off[nseg] = n, non-decreasing)
+ *   fobs (n,) observed fluxes, c_inv (n,) the diagonal of the inverse covariance
+ *   ll (B,) out: sum over segments of -chi2/2 + log det(2 pi Sigma)/2 (linalg.py:68)
+ *   beta (B, nseg, 2) out or NULL: the least-squares (F_s, F_b) of each segment
+ *   g_A (B, n) out or NULL, row stride ldg >= n: g_ll[b] * d ll_b / d A_bi (g_ll NULL:
+ *     cotangent 1); the gradient the reference's autodiff gives with respect to A
+ * A segment whose normal matrix is singular (fewer than two distinct magnifications)
+ * gives a non-finite ll, as the reference's solve does.  B <= 2^26. */
+int caus_marginalized_log_likelihood(void* stream, const double* A, int64_t B, int64_t n,
+                                     int64_t lda, const double* fobs, const double* c_inv,
+                                     const int64_t* seg_off, int32_t nseg, double* ll,
+                                     double* beta, const double* g_ll, double* g_A, int64_t ldg);
+
 #ifdef __cplusplus
 }
 #endif
