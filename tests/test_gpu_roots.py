@@ -87,6 +87,28 @@ def test_all_degrees_known_roots(gpu, deg):
         assert ok.all()
 
 
+def test_persistent_refill_large_batch(gpu):
+    # more polynomials than the persistent kernel has lanes (8 workgroups of 256 per
+    # compute unit), so lanes start second polynomials; known roots
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    n = 8 * cus * 256 + 12345
+    g = np.random.default_rng(7)
+    zs = g.normal(size=(n, 5)) + 1j * g.normal(size=(n, 5))
+    c = np.zeros((n, 6), complex)
+    c[:, 0] = 1.0
+    for k in range(5):   # highest-first product (z - r_0)...(z - r_k)
+        c[:, 1:k + 2] = c[:, 1:k + 2] - zs[:, k:k + 1] * c[:, 0:k + 1]
+    c = c[:, ::-1].copy()   # ascending
+    z, status, _ = solve_with_stats(c, mode="fp64", init="complex")
+    z = z.cpu().numpy()
+    assert status.cpu().numpy().all()
+    assert np.max(residual_ratio(c, z)) < 1e-13
+    pick = np.concatenate([np.arange(200), n - 200 + np.arange(200),
+                           g.choice(n, 2000, replace=False)])
+    ok, _, _ = roots_close(z[pick], zs[pick], c[pick], rel=1e-10)
+    assert ok.all()
+
+
 def test_custom_init(gpu):
     c = lens_polys(2, 4000, seed=5)
     z0 = ea.roots_asc(c)
