@@ -109,7 +109,7 @@ def main():
     # stream and the degree-5 batch on another, so that each batch's tail (its last,
     # slowest polynomials) overlaps the other's work.  Each kernel's own duration, for the
     # roofline, is timed separately with HIP events around single launches on one stream.
-    s_a, s_b = torch.cuda.Stream(), torch.cuda.Stream()   # default priority
+    s_a, s_b = _lib.side_streams(2)   # the library's pool (default priority)
     results = {}
     for name, (mode, init) in forms.items():
         steps = args.steps if name == "fp64" else max(3, args.steps // 4)

@@ -90,6 +90,20 @@ def lib():
     return L
 
 
+_SIDE_STREAMS = {}
+
+
+def side_streams(n=2):
+    """The library's pool of extra default-priority streams on the current device,
+    created once and shared by every caller: a process has few hardware queues (4 by
+    default), and two streams that share one run their work one after the other."""
+    dev = torch.cuda.current_device()
+    got = _SIDE_STREAMS.setdefault(dev, [])
+    while len(got) < n:
+        got.append(torch.cuda.Stream())
+    return got[:n]
+
+
 def stream_ptr(device=None):
     return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
 

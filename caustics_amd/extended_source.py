@@ -37,10 +37,14 @@ def _col(x, n, dtype=torch.float64):
 
 
 def _run(nlenses, w, rho_t, prm, npts_t, npts_max, ld, u1_t, npts_ld, itmax, mode_id, g=None,
-         max_chunk=None, work=None):
-    """One or more launches of the contour pipeline; `work`, a (5,) int64 CUDA tensor,
-    accumulates the kernels' work counters (FP64, FP32 and compensated root-iterations,
-    limb-point solves, contour points)."""
+         max_chunk=None, work=None, streams=2):
+    """One or more launch sequences of the contour pipeline, one per chunk of sources.
+    With several chunks they alternate over `streams` streams, each with its own
+    workspace, so that one chunk's low-occupancy launches (refinement rounds,
+    continuation, stitching) overlap another chunk's solves and integrals; the caller's
+    stream waits for all of them.  `work`, a (5,) int64 CUDA tensor, accumulates the
+    kernels' work counters (FP64, FP32 and compensated root-iterations, limb-point
+    solves, contour points)."""
     L = _lib.lib()
     S = w.numel()
     P = 0 if prm is None else prm.shape[1]
@@ -51,19 +55,36 @@ def _run(nlenses, w, rho_t, prm, npts_t, npts_max, ld, u1_t, npts_ld, itmax, mod
     if per < 0:
         raise ValueError(f"npts_limb={npts_max} out of range (4..4096)")
     chunk = max_chunk or max(1, min(S, (2 << 30) // per))
-    ws = torch.empty((chunk * per + 7) // 8, dtype=torch.float64, device="cuda")
-    for lo in range(0, S, chunk):
+    nchunks = (S + chunk - 1) // chunk
+    nstr = max(1, min(int(streams), nchunks))
+    main = torch.cuda.current_stream()
+    lanes = [main] if nstr == 1 else _lib.side_streams(nstr)
+    ws = [torch.empty((chunk * per + 7) // 8, dtype=torch.float64, device="cuda")
+          for _ in range(nstr)]
+    for st in lanes:
+        if st is not main:
+            st.wait_stream(main)   # inputs and outputs were produced on the caller's stream
+    for ci, lo in enumerate(range(0, S, chunk)):
         hi = min(S, lo + chunk)
         sl = slice(lo, hi)
-        rc = L.caus_mag_extended_source_work(
-            _lib.stream_ptr(), nlenses, _lib.ptr(w[sl]), hi - lo, _lib.ptr(rho_t[sl]),
-            None if prm is None else _lib.ptr(prm[sl]), P,
-            None if npts_t is None else _lib.ptr(npts_t[sl]), npts_max, 1 if ld else 0,
-            None if u1_t is None else _lib.ptr(u1_t[sl]), 1, int(npts_ld),
-            None if g is None else _lib.ptr(g[sl]), _lib.ptr(mag[sl]),
-            None if g_out is None else _lib.ptr(g_out[sl]), _lib.ptr(status[sl]), _lib.ptr(ws),
-            ws.numel() * 8, int(itmax), mode_id, None if work is None else _lib.ptr(work))
+        k = ci % nstr
+        with torch.cuda.stream(lanes[k]):
+            rc = L.caus_mag_extended_source_work(
+                _lib.stream_ptr(), nlenses, _lib.ptr(w[sl]), hi - lo, _lib.ptr(rho_t[sl]),
+                None if prm is None else _lib.ptr(prm[sl]), P,
+                None if npts_t is None else _lib.ptr(npts_t[sl]), npts_max, 1 if ld else 0,
+                None if u1_t is None else _lib.ptr(u1_t[sl]), 1, int(npts_ld),
+                None if g is None else _lib.ptr(g[sl]), _lib.ptr(mag[sl]),
+                None if g_out is None else _lib.ptr(g_out[sl]), _lib.ptr(status[sl]),
+                _lib.ptr(ws[k]), ws[k].numel() * 8, int(itmax), mode_id,
+                None if work is None else _lib.ptr(work))
         _lib.check(rc, "caus_mag_extended_source")
+    for st in lanes:
+        if st is not main:
+            main.wait_stream(st)   # every later use of the outputs and workspaces is ordered
+            for t in [*ws, mag, status, g_out, w, rho_t, prm, npts_t, u1_t, g, work]:
+                if t is not None:
+                    t.record_stream(st)   # no reuse of their memory before the side work ends
     return mag, status, g_out
 
 
