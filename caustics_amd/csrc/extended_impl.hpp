@@ -444,9 +444,10 @@ __device__ __forceinline__ double sqrt_rsq(double x, double& inv) {
 // B alike.
 template <int NL>
 __device__ __forceinline__ double ld_brightness_core(cd z, const LensFrame<NL>& lf, cd w0,
-                                                     double irho, double u1, double c3,
-                                                     cd (&v)[NL], cd& D, double& iaD,
-                                                     double& r, double& Br, double& dBr) {
+                                                     double rho, double irho, double u1,
+                                                     double c3, cd (&v)[NL], cd& D,
+                                                     double& iaD, double& r, double& Br,
+                                                     double& dBr) {
 #pragma clang fp contract(off)
     const cd zb = conj(z);
     cd G = cd{0.0, 0.0};
@@ -463,8 +464,7 @@ __device__ __forceinline__ double ld_brightness_core(cd z, const LensFrame<NL>& 
     iaD = pos ? ia : 0.0;
     r = pos ? aD * irho : 0.0;
     const bool inside = r <= 1.0;
-    double q = r;
-    if (!inside) q = rcp(r);
+    const double q = inside ? r : rho * iaD;   // 1/r = rho / |D|
     const double x = 1.0 - q * q;
     double is;
     const double sq = sqrt_rsq(x, is);
@@ -480,11 +480,11 @@ __device__ __forceinline__ double ld_brightness_core(cd z, const LensFrame<NL>& 
 __device__ __forceinline__ double ld_c3(double u1) { return 3.0 / (3.0 - u1); }
 
 template <int NL>
-__device__ __forceinline__ double ld_brightness(cd z, const LensFrame<NL>& lf, cd w0, double irho,
-                                                double u1, double c3) {
+__device__ __forceinline__ double ld_brightness(cd z, const LensFrame<NL>& lf, cd w0, double rho,
+                                                double irho, double u1, double c3) {
     cd v[NL], D;
     double iaD, r, Br, dBr;
-    return ld_brightness_core<NL>(z, lf, w0, irho, u1, c3, v, D, iaD, r, Br, dBr);
+    return ld_brightness_core<NL>(z, lf, w0, rho, irho, u1, c3, v, D, iaD, r, Br, dBr);
 }
 
 // one trapezoid term of the limb-darkened contour integral (integrate.py:118-121),
@@ -523,7 +523,7 @@ __device__ __forceinline__ double ld_line(bool isP, double fixed, double a, doub
         for (int i = 0; i < n; ++i) {
             const double v = fma(h, x[i], c);
             const cd zz = isP ? cd{fixed, v} : cd{v, fixed};
-            s = fma(h * ld_brightness<NL>(zz, lf, w0, irho, u1, c3), wt[i], s);
+            s = fma(h * ld_brightness<NL>(zz, lf, w0, rho, irho, u1, c3), wt[i], s);
         }
         I += s;
     }
@@ -749,12 +749,12 @@ struct LDAdj {
 // dD = -(dG + dw0),  dG = sum_j de_j v_j + e_j v_j^2 conj(dr_j),  v_j = 1/(conj z - conj r_j).
 // dc3 = dc3/du1 = c3^2/3.
 template <int NL>
-__device__ inline double ld_brightness_adj(cd z, const LensFrame<NL>& lf, cd w0, double irho,
-                                           double u1, double c3, double dc3, double c, double& Bx,
-                                           double& By, LDAdj<NL>& A) {
+__device__ inline double ld_brightness_adj(cd z, const LensFrame<NL>& lf, cd w0, double rho,
+                                           double irho, double u1, double c3, double dc3, double c,
+                                           double& Bx, double& By, LDAdj<NL>& A) {
     cd v[NL], D;
     double iaD, r, Br, dBr;
-    const double B = ld_brightness_core<NL>(z, lf, w0, irho, u1, c3, v, D, iaD, r, Br, dBr);
+    const double B = ld_brightness_core<NL>(z, lf, w0, rho, irho, u1, c3, v, D, iaD, r, Br, dBr);
     cd vv[NL];
 #pragma unroll
     for (int j = 0; j < NL; ++j) vv[j] = v[j] * v[j];
@@ -832,8 +832,8 @@ __device__ inline double ld_line_adj(bool isP, double fixed, double a, double b,
             const double t = fma(h, xi, c);
             const cd zz = isP ? cd{fixed, t} : cd{t, fixed};
             double Bx, By;
-            const double B = ld_brightness_adj<NL>(zz, lf, w0, irho, u1, c3, dc3, cw * wi, Bx, By,
-                                                   A);
+            const double B = ld_brightness_adj<NL>(zz, lf, w0, rho, irho, u1, c3, dc3, cw * wi, Bx,
+                                                   By, A);
             const double fl = isP ? By : Bx;   // along the line
             const double ff = isP ? Bx : By;   // across (fixed coordinate)
             s = fma(h * B, wi, s);
