@@ -188,8 +188,10 @@ def main():
         for key, name, fn in (
                 ("c1", "point_source_1e4", lambda: bench_point_source(args.steps, 2)),
                 ("c3", "lightcurves_256x2000_fwd_vjp", lambda: bench_lightcurves(3, 1)),
-                ("c4", "extended_binary_ld_4096_fwd_vjp", lambda: bench_extended_binary(3, 1)),
-                ("c5", "triple_extended_lightcurves_fwd_vjp", lambda: bench_triple(1, 0)),
+                ("c4", "extended_binary_ld_4096_fwd_vjp",
+                 lambda: bench_extended_binary(3, 1, cpu_s=0 if args.no_cpu else args.cpu_seconds)),
+                ("c5", "triple_extended_lightcurves_fwd_vjp",
+                 lambda: bench_triple(1, 0, cpu_s=0 if args.no_cpu else args.cpu_seconds)),
                 ("ll", "marginalized_log_likelihood_grad", lambda: bench_likelihood(20, 3))):
             if key not in sel:
                 continue
@@ -345,7 +347,32 @@ def _roofline_entry(flops, dt):
             "achieved_fp32_tflops": f32 / dt / 1e12, "frac": busy}
 
 
-def bench_extended_binary(steps, warmup, n_near=2048, n_far=2048):
+def cpu_extended_baseline(items, budget_s, what):
+    """The oracle's mag_extended_source (NumPy restatement + the C root oracle) on one
+    thread, over `items` (keyword dicts) in order until budget_s is spent.  Forward only:
+    the oracle has no reverse mode, so this is a lower bound on the CPU cost of the
+    fwd+VJP evaluations the GPU figure counts."""
+    import ctypes
+
+    from oracle import extended as OE
+    try:
+        ctypes.CDLL("libgomp.so.1").omp_set_num_threads(1)   # the C oracle's solves
+    except OSError:
+        pass
+    n = 0
+    t0 = time.perf_counter()
+    for kw in items:
+        OE.mag_extended_source(**kw)
+        n += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    print(f"bench: CPU oracle {what}: {n} evals in {dt:.1f} s", file=sys.stderr, flush=True)
+    return {"value": n / dt, "unit": "mag evals/s (forward only)", "cores": 1, "kind": "port",
+            "sample": f"{n} {what}, forward only, {dt:.1f} s"}
+
+
+def bench_extended_binary(steps, warmup, n_near=2048, n_far=2048, cpu_s=0.0):
     """Binary limb-darkened (u=0.5) sources, rho log-uniform [1e-3,1e-1]: n_near within
     2 rho of a caustic with ragged contour resolution 256..4096 points, n_far accepted by
     the hexadecapole gate.  One step is mag() forward + VJP: the gate on all sources,
@@ -396,10 +423,17 @@ def bench_extended_binary(steps, warmup, n_near=2048, n_far=2048):
                         "p99_rel_err_vs_fp64": float(np.quantile(e, 0.99)),
                         "frac_le_1e-8": float(np.mean(e <= 1e-8)),
                         "grad_p99_rel_err_vs_fp64": float(np.quantile(ge, 0.99))})
+    if cpu_s > 0:
+        items = (dict(w0=complex(b["w"][i]), rho=float(b["rho"][i]), nlenses=2,
+                      npts_limb=int(b["npts"][i]), limb_darkening=True, u1=float(b["u1"][i]),
+                      npts_ld=100, s=float(b["s"][i]), q=float(b["q"][i]))
+                 for i in range(n_near))
+        out["cpu_baseline"] = cpu_extended_baseline(
+            items, cpu_s, "near-caustic C4 sources (their ragged npts_limb)")
     return out
 
 
-def bench_triple(steps, warmup, n_sets=1024, n_epochs=1500, npts=256, budget_s=40.0):
+def bench_triple(steps, warmup, n_sets=1024, n_epochs=1500, npts=256, budget_s=40.0, cpu_s=0.0):
     """Triple-lens caustic-crossing light curves (u=0.4, rho log-uniform [1e-3,3e-2]),
     full limb-darkened contour at every epoch, fused fwd+VJP, both solver forms.  The
     timed batch is the full n_sets x n_epochs when one step fits budget_s, otherwise the
@@ -452,6 +486,16 @@ def bench_triple(steps, warmup, n_sets=1024, n_epochs=1500, npts=256, budget_s=4
     out["cf32"].update({"max_rel_err_vs_fp64": float(e.max()),
                         "p99_rel_err_vs_fp64": float(np.quantile(e, 0.99)),
                         "frac_le_1e-8": float(np.mean(e <= 1e-8))})
+    if cpu_s > 0:
+        # every 25th epoch of the parameter sets in order, so the sample spans whole
+        # light curves (caustic crossings included)
+        items = (dict(w0=complex(b["w"][k][e]), rho=float(b["rho"][k]), nlenses=3,
+                      npts_limb=npts, limb_darkening=True, u1=float(b["u1"][k]), npts_ld=100,
+                      s=float(b["s"][k]), q=float(b["q"][k]), q3=float(b["q3"][k]),
+                      r3=float(b["r3"][k]), psi=float(b["psi"][k]))
+                 for k in range(n_sets) for e in range(0, n_epochs, 25))
+        out["cpu_baseline"] = cpu_extended_baseline(
+            items, cpu_s, "C5 epochs (every 25th epoch of the light curves in order)")
     return out
 
 
