@@ -62,6 +62,8 @@ def main():
     ap.add_argument("--roots-only", action="store_true")
     ap.add_argument("--workloads", default="c1,c3,c4,c5,ll",
                     help="comma-separated extra workloads to run after the root solve")
+    ap.add_argument("--c5-sets", type=int, default=1024,
+                    help="parameter sets of the C5 workload (1024 = the full workload)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a kernel of "
                          "known traffic, to scale hardware byte counters)")
@@ -191,7 +193,8 @@ def main():
                 ("c4", "extended_binary_ld_4096_fwd_vjp",
                  lambda: bench_extended_binary(3, 1, cpu_s=0 if args.no_cpu else args.cpu_seconds)),
                 ("c5", "triple_extended_lightcurves_fwd_vjp",
-                 lambda: bench_triple(1, 0, cpu_s=0 if args.no_cpu else args.cpu_seconds)),
+                 lambda: bench_triple(1, 0, n_sets=args.c5_sets,
+                                      cpu_s=0 if args.no_cpu else args.cpu_seconds)),
                 ("ll", "marginalized_log_likelihood_grad", lambda: bench_likelihood(20, 3))):
             if key not in sel:
                 continue
@@ -312,6 +315,24 @@ def bench_lightcurves(steps, warmup, host_check=True):
         torch.cuda.synchronize()
         mags[name] = (mag.clone(), gp.clone())
         out[name] = {"mag_evals_per_s_fwd_vjp": L_ * T / dt, "ms": dt * 1e3}
+    # roofline: the kernel solves the epochs' lens polynomials with the same solver and
+    # initial estimates as caus_roots, so the root-iterations of each form are counted
+    # by solving the same polynomials with the counting entry point
+    from caustics_amd.ehrlich_aberth_primitive import solve_with_stats
+    tt = t.cpu().numpy()[None, :]
+    pr = {k: np.asarray(prm[k], float)[:, None] for k in ("t0", "u0", "tE", "alpha", "s", "q")}
+    w_user = np.exp(-1j * pr["alpha"]) * (pr["u0"] + 1j * (tt - pr["t0"]) / pr["tE"])
+    lf, x_cm = W.lens_frame({"s": pr["s"], "q": pr["q"]}, 2)
+    c = W.coeffs_product((w_user + x_cm).reshape(-1), np.broadcast_to(lf["a"], w_user.shape).reshape(-1),
+                         np.broadcast_to(lf["e1"], w_user.shape).reshape(-1), None, None, 2)
+    for name in ("fp64", "cf32"):
+        _, _, st = solve_with_stats(c, mode=name, init="complex")
+        sw = st.sum(0).cpu().numpy().astype(np.float64)
+        work = [sw[1], sw[3], sw[2], float(L_ * T), 0.0]
+        out[name]["roofline"] = _roofline_entry(contour_flops(work, 2, False),
+                                                out[name]["ms"] * 1e-3)
+        out[name]["roofline"]["rule"] = ("root-iterations x 160 + per epoch 1e3 (coefficients) "
+                                         "+ 5 x 65 (lens equation, det J); VJP not counted")
     m64, g64 = mags["fp64"]
     m32, g32 = mags["cf32"]
     rel = ((m32 - m64).abs() / m64.abs()).cpu().numpy()
