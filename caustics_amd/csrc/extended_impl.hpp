@@ -181,7 +181,7 @@ __device__ __forceinline__ bool ext_solve_point(cd w, const LensFrame<NL>& lf,
             if (norm2(z[j]) < 1.0) flags |= 1u << (16 + j);
         }
     } else {
-        ok = solve_lens<NL, MODE>(w, lf, z, itmax, false, stats);
+        ok = solve_lens<NL, MODE, true>(w, lf, z, itmax, false, stats);
 #pragma unroll
         for (int j = 0; j < DEG; ++j) {
             ImageTerms<NL> it = image_terms<NL>(z[j], lf);

@@ -22,7 +22,8 @@ constexpr int ps_degree() {
 }
 
 // warm = true: z holds initial estimates (the roots of a nearby source position)
-template <int NL, int MODE>
+// POLISH: one compensated Newton step per root after convergence (polish_roots).
+template <int NL, int MODE, bool POLISH = false>
 __device__ __forceinline__ bool solve_lens(cd wl, const LensFrame<NL>& lf, cd (&z)[ps_degree<NL>()],
                                            int itmax, bool warm = false,
                                            SolveStats* stats = nullptr) {
@@ -35,6 +36,7 @@ __device__ __forceinline__ bool solve_lens(cd wl, const LensFrame<NL>& lf, cd (&
         ok = solve_cf32<DEG>(p, z, itmax, warm, st);
     else
         ok = solve_fp64<DEG, MODE, false>(p, z, itmax, warm, st);
+    if constexpr (POLISH) polish_roots<DEG>(p, z);
     if (stats) {
         stats->sweeps += st.sweeps;
         stats->it64 += st.it64;

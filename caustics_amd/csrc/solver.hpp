@@ -203,6 +203,43 @@ __device__ __forceinline__ void horner_comp(const cd (&p)[DEG + 1], cd x, cd& h,
     hd = hd + ed;
 }
 
+// Value of the polynomial by compensated Horner (TwoProd/TwoSum error terms summed in
+// plain FP64, accurate as if computed in twice the working precision) and its derivative
+// by plain Horner.
+template <int DEG>
+__device__ __forceinline__ void horner_comp_value(const cd (&p)[DEG + 1], cd x, cd& h, cd& hd) {
+#pragma clang fp contract(off)
+    cd e = cd{0, 0};
+    h = p[DEG];
+    hd = cd{0, 0};
+#pragma unroll
+    for (int i = DEG - 1; i >= 0; --i) {
+        hd = cd{fma(hd.re, x.re, fma(-hd.im, x.im, h.re)), fma(hd.re, x.im, fma(hd.im, x.re, h.im))};
+        cd r, e1, e2, e3, se;
+        two_prod_c(h, x, r, e1, e2, e3);
+        two_sum_c(r, p[i], h, se);
+        const cd t = cd{(e1.re + e2.re) + (e3.re + se.re), (e1.im + e2.im) + (e3.im + se.im)};
+        e = cd{fma(e.re, x.re, fma(-e.im, x.im, t.re)), fma(e.re, x.im, fma(e.im, x.re, t.im))};
+    }
+    h = h + e;
+}
+
+// One Newton step per root with the compensated residual.  For a root that met the
+// stopping test, the step is far below an ulp of the root and is computed to a small
+// fraction of one, so the result is (nearly always) the double nearest the exact root:
+// roots that different iterations (FP64 or CF32, other initial estimates) left a few
+// ulps apart become the same doubles, and decisions taken on them downstream agree.
+// Clustered roots (near a caustic) can still differ.
+template <int DEG>
+__device__ __forceinline__ void polish_roots(const cd (&p)[DEG + 1], cd (&z)[DEG]) {
+#pragma unroll
+    for (int j = 0; j < DEG; ++j) {
+        cd h, hd;
+        horner_comp_value<DEG>(p, z[j], h, hd);
+        if (norm2(hd) > 0.0) z[j] = z[j] - cdiv<2>(h, hd);
+    }
+}
+
 // One compensated-stage step for root j (ehrlich_aberth.h:226-280).  Returns true
 // when the root is declared converged at the compensated level.
 template <int DEG, bool REFCOMPAT>
