@@ -114,8 +114,18 @@ def test_cf32_form_matches_fp64(gpu, nlenses):
     e = np.abs(m32 - m64) / m64
     eg = np.abs(g32 - g64) / np.maximum(np.abs(g64), 1e-3 * m64[:, None])
     print(f"cf32 vs fp64 nl={nlenses}: mag max rel {e.max():.2e}, grad max rel {eg.max():.2e}")
-    assert np.mean(e <= 1e-8) >= 0.95 and e.max() < 1e-3
-    assert np.mean(eg <= 1e-6) >= 0.95
+    # both forms end each limb solve with the same compensated Newton step, which takes
+    # converged roots to the doubles nearest the exact roots: the contours coincide
+    assert e.max() <= 1e-8
+    assert eg.max() <= 1e-6
+    # a larger batch across caustic crossings: clustered images may still differ
+    w2 = caustic_points(rho, nlenses=nlenses, npts=40, **prm)
+    w2 = w2 + rho * 0.7 * np.exp(1j * np.linspace(0, 20, len(w2)))
+    m64, _, _ = extended_fwd_vjp(w2, rho, np.ones(len(w2)), mode="fp64", **kw)
+    m32, _, _ = extended_fwd_vjp(w2, rho, np.ones(len(w2)), mode="cf32", **kw)
+    e2 = ((m32 - m64).abs() / m64).cpu().numpy()
+    print(f"  {len(w2)} sources: frac <= 1e-8 {np.mean(e2 <= 1e-8):.4f}")
+    assert np.mean(e2 <= 1e-8) >= 0.98
 
 
 def test_ragged_batch_matches_individual(gpu):
