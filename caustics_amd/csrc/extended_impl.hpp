@@ -1365,11 +1365,12 @@ __global__ __launch_bounds__(kExtWaveBlock) void k_ext_chain(ExtArgs A, int cur,
 // image gaps (bitonic sort of the gap keys in LDS, descending by (gap, position)),
 // insertion shifts, the next limb order and the carried-over gap keys, one task per new
 // point.  Points stay where they are (append order); only the order and keys move.
-template <int NL>
-__global__ __launch_bounds__(kExtWaveBlock) void k_ext_select(ExtArgs A, int cur, int round) {
+// NT threads per source: one wave for short contours, four when the sort is long.
+template <int NL, int NT>
+__global__ __launch_bounds__(NT) void k_ext_select(ExtArgs A, int cur, int round) {
     constexpr int DEG = ext_degree<NL>();
     extern __shared__ uint64_t s_dyn[];
-    __shared__ int s_int[kExtWaveBlock];
+    __shared__ int s_int[NT];
     const int tid = threadIdx.x;
     const int64_t src = blockIdx.x;
     const ExtBatch B = ext_batch(A);
@@ -1383,14 +1384,14 @@ __global__ __launch_bounds__(kExtWaveBlock) void k_ext_select(ExtArgs A, int cur
     while (P2 < M - 1) P2 <<= 1;
     uint64_t* s_key = s_dyn;
     int* s_idx = reinterpret_cast<int*>(s_dyn + P2);
-    for (int k = tid; k < P2; k += kExtWaveBlock) {
+    for (int k = tid; k < P2; k += NT) {
         s_key[k] = (k < M - 1) ? gk[k] : 0ull;
         s_idx[k] = (k < M - 1) ? k : -1;
     }
     __syncthreads();
     for (int size = 2; size <= P2; size <<= 1) {
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int i = tid; i < P2; i += kExtWaveBlock) {
+            for (int i = tid; i < P2; i += NT) {
                 const int j = i ^ stride;
                 if (j > i) {
                     const bool desc = (i & size) == 0;
@@ -1410,30 +1411,30 @@ __global__ __launch_bounds__(kExtWaveBlock) void k_ext_select(ExtArgs A, int cur
     }
     // selected gaps -> exclusive prefix count = shift of every point
     int* shift = reinterpret_cast<int*>(W.perm);
-    const int per = (M + kExtWaveBlock - 1) / kExtWaveBlock;
+    const int per = (M + NT - 1) / NT;
     const int k0 = tid * per, k1 = min(M, k0 + per);
     for (int k = k0; k < k1; ++k) shift[k] = 0;
     __syncthreads();
-    for (int c = tid; c < nadd; c += kExtWaveBlock) shift[s_idx[c]] = 1;
+    for (int c = tid; c < nadd; c += NT) shift[s_idx[c]] = 1;
     __syncthreads();
     int cnt = 0;
     for (int k = k0; k < k1; ++k) cnt += shift[k];
     int total;
-    int run = block_exclusive_scan<kExtWaveBlock>(cnt, s_int, total);
+    int run = block_exclusive_scan<NT>(cnt, s_int, total);
     for (int k = k0; k < k1; ++k) {
         const int ins = shift[k];
         shift[k] = run | (ins << 30);   // bit 30: the gap after point k is refined
         run += ins;
     }
     __syncthreads();
-    for (int k = tid; k < M; k += kExtWaveBlock) {
+    for (int k = tid; k < M; k += NT) {
         const int sh = shift[k];
         const int d = k + (sh & 0x3FFFFFFF);
         W.ord[nxt][d] = ord[k];
         if (k + 1 < M && !(sh >> 30)) W.gk[nxt][d] = gk[k];
     }
     const int64_t t0 = B.off_add[src];
-    for (int c = tid; c < nadd; c += kExtWaveBlock) {
+    for (int c = tid; c < nadd; c += NT) {
         const int k = s_idx[c];
         const int d = k + (shift[k] & 0x3FFFFFFF) + 1;
         const int a = ord[k], b = ord[k + 1];
@@ -1915,9 +1916,14 @@ int ext_run(hipStream_t s, const ExtArgs& a) {
     const size_t lds = (size_t)P2 * (sizeof(uint64_t) + sizeof(int));
     const int add_grid = ext_flat_grid(S * (int64_t)(a.nadd_max > 0 ? a.nadd_max : 1));
     int cur = 0;
+    const bool wide = P2 > 512;   // long contours: four waves share each sort
     for (int round = 0; round < 10; ++round) {
-        hipLaunchKernelGGL((k_ext_select<NL>), dim3(nb), dim3(kExtWaveBlock), lds, s, a, cur,
-                           round);
+        if (wide)
+            hipLaunchKernelGGL((k_ext_select<NL, kExtBlock>), dim3(nb), dim3(kExtBlock), lds, s, a,
+                               cur, round);
+        else
+            hipLaunchKernelGGL((k_ext_select<NL, kExtWaveBlock>), dim3(nb), dim3(kExtWaveBlock),
+                               lds, s, a, cur, round);
         hipLaunchKernelGGL((k_ext_solve_add<NL, MODE>), dim3(add_grid), dim3(kExtFlatBlock), 0, s,
                            a, cur);
         cur ^= 1;

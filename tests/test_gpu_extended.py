@@ -128,6 +128,17 @@ def test_cf32_form_matches_fp64(gpu, nlenses):
     assert np.mean(e2 <= 1e-8) >= 0.98
 
 
+def test_long_contours_match_oracle(gpu):
+    # npts_limb above 512: the gap sort of a refinement round runs on four waves
+    rho = 2e-2
+    w = caustic_points(rho, npts=4, s=0.8, q=0.5)[:12]
+    m = mag_extended_source(w, rho, nlenses=2, npts_limb=1500, limb_darkening=True, u1=0.5,
+                            s=0.8, q=0.5).cpu().numpy()
+    mo = np.array([E.mag_extended_source(x, rho, nlenses=2, npts_limb=1500, limb_darkening=True,
+                                         u1=0.5, s=0.8, q=0.5) for x in w])
+    compare(m, mo, "binary LD npts_limb=1500")
+
+
 def test_ragged_batch_matches_individual(gpu):
     rho = 2e-2
     w = caustic_points(rho, npts=5, s=1.1, q=0.3)
