@@ -39,40 +39,64 @@ def _col(x, n, dtype=torch.float64):
 def _run(nlenses, w, rho_t, prm, npts_t, npts_max, ld, u1_t, npts_ld, itmax, mode_id, g=None,
          max_chunk=None, work=None, streams=2):
     """One or more launch sequences of the contour pipeline, one per chunk of sources.
-    With several chunks they alternate over `streams` streams, each with its own
-    workspace, so that one chunk's low-occupancy launches (refinement rounds,
-    continuation, stitching) overlap another chunk's solves and integrals; the caller's
-    stream waits for all of them.  `work`, a (5,) int64 CUDA tensor, accumulates the
-    kernels' work counters (FP64, FP32 and compensated root-iterations, limb-point
-    solves, contour points)."""
+    A ragged batch (per-source npts_limb) is processed in descending npts_limb order, so
+    that every launch holds sources of similar cost (no workgroup of a long contour
+    trailing a launch of short ones) and each chunk's workspace is sized by its own
+    largest contour; results return in the caller's order.  With several chunks they
+    alternate over `streams` streams, each with its own workspace, so that one chunk's
+    low-occupancy launches (refinement rounds, continuation, stitching) overlap another
+    chunk's solves and integrals; the caller's stream waits for all of them.  Per-source
+    results do not depend on the batch around them.  `work`, a (5,) int64 CUDA tensor,
+    accumulates the kernels' work counters (FP64, FP32 and compensated root-iterations,
+    limb-point solves, contour points)."""
     L = _lib.lib()
     S = w.numel()
     P = 0 if prm is None else prm.shape[1]
+    if extended_workspace_bytes(nlenses, 1, npts_max) < 0:
+        raise ValueError(f"npts_limb={npts_max} out of range (4..4096)")
+    order = None
+    if npts_t is not None and S > 1:
+        eff = np.clip(npts_t.detach().cpu().numpy().astype(np.int64), 4, npts_max)
+        order_np = np.argsort(-eff, kind="stable")
+        eff = eff[order_np]
+        if not np.array_equal(order_np, np.arange(S)):
+            order = torch.as_tensor(order_np, device="cuda")
+            pick = lambda x: None if x is None else x[order].contiguous()
+            w, rho_t, prm, npts_t, u1_t, g = (pick(w), pick(rho_t), pick(prm), pick(npts_t),
+                                              pick(u1_t), pick(g))
+    else:
+        eff = None
     mag = torch.empty(S, dtype=torch.float64, device="cuda")
     status = torch.empty(S, dtype=torch.uint8, device="cuda")
     g_out = torch.empty((S, 4 + P), dtype=torch.float64, device="cuda") if g is not None else None
-    per = extended_workspace_bytes(nlenses, 1, npts_max)
-    if per < 0:
-        raise ValueError(f"npts_limb={npts_max} out of range (4..4096)")
-    chunk = max_chunk or max(1, min(S, (2 << 30) // per))
-    nchunks = (S + chunk - 1) // chunk
-    nstr = max(1, min(int(streams), nchunks))
+    # chunks (lo, hi, largest npts_limb of the chunk)
+    chunks = []
+    lo = 0
+    while lo < S:
+        nm = npts_max if eff is None else int(eff[lo])
+        per = extended_workspace_bytes(nlenses, 1, nm)
+        cnt = max_chunk or max(1, (2 << 30) // per)
+        hi = min(S, lo + cnt)
+        chunks.append((lo, hi, nm))
+        lo = hi
+    nstr = max(1, min(int(streams), len(chunks)))
     main = torch.cuda.current_stream()
     lanes = [main] if nstr == 1 else _lib.side_streams(nstr)
-    ws = [torch.empty((chunk * per + 7) // 8, dtype=torch.float64, device="cuda")
-          for _ in range(nstr)]
+    need = [0] * nstr
+    for ci, (lo, hi, nm) in enumerate(chunks):
+        need[ci % nstr] = max(need[ci % nstr], extended_workspace_bytes(nlenses, hi - lo, nm))
+    ws = [torch.empty((b + 7) // 8, dtype=torch.float64, device="cuda") for b in need]
     for st in lanes:
         if st is not main:
             st.wait_stream(main)   # inputs and outputs were produced on the caller's stream
-    for ci, lo in enumerate(range(0, S, chunk)):
-        hi = min(S, lo + chunk)
+    for ci, (lo, hi, nm) in enumerate(chunks):
         sl = slice(lo, hi)
         k = ci % nstr
         with torch.cuda.stream(lanes[k]):
             rc = L.caus_mag_extended_source_work(
                 _lib.stream_ptr(), nlenses, _lib.ptr(w[sl]), hi - lo, _lib.ptr(rho_t[sl]),
                 None if prm is None else _lib.ptr(prm[sl]), P,
-                None if npts_t is None else _lib.ptr(npts_t[sl]), npts_max, 1 if ld else 0,
+                None if npts_t is None else _lib.ptr(npts_t[sl]), nm, 1 if ld else 0,
                 None if u1_t is None else _lib.ptr(u1_t[sl]), 1, int(npts_ld),
                 None if g is None else _lib.ptr(g[sl]), _lib.ptr(mag[sl]),
                 None if g_out is None else _lib.ptr(g_out[sl]), _lib.ptr(status[sl]),
@@ -85,6 +109,9 @@ def _run(nlenses, w, rho_t, prm, npts_t, npts_max, ld, u1_t, npts_ld, itmax, mod
             for t in [*ws, mag, status, g_out, w, rho_t, prm, npts_t, u1_t, g, work]:
                 if t is not None:
                     t.record_stream(st)   # no reuse of their memory before the side work ends
+    if order is not None:
+        back = lambda x: None if x is None else torch.empty_like(x).index_copy_(0, order, x)
+        mag, status, g_out = back(mag), back(status), back(g_out)
     return mag, status, g_out
 
 
