@@ -202,9 +202,9 @@ def main():
         sel = set(args.workloads.split(","))
         for key, name, fn in (
                 ("c1", "point_source_1e4", lambda: bench_point_source(args.steps, 2)),
-                ("c3", "lightcurves_256x2000_fwd_vjp", lambda: bench_lightcurves(3, 1)),
+                ("c3", "lightcurves_256x2000_fwd_vjp", lambda: bench_lightcurves(20, 5)),
                 ("c4", "extended_binary_ld_4096_fwd_vjp",
-                 lambda: bench_extended_binary(3, 1, cpu_s=0 if args.no_cpu else args.cpu_seconds)),
+                 lambda: bench_extended_binary(5, 2, cpu_s=0 if args.no_cpu else args.cpu_seconds)),
                 ("c5", "triple_extended_lightcurves_fwd_vjp",
                  lambda: bench_triple(1, 0, n_sets=args.c5_sets,
                                       cpu_s=0 if args.no_cpu else args.cpu_seconds)),
