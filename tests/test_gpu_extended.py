@@ -6,6 +6,7 @@ near a threshold may differ: those stay within the discretisation error, 1e-3), 
 within the reference tests' own rtol 1e-3 of the single-lens closed forms."""
 import numpy as np
 import pytest
+import torch
 from scipy.integrate import quad
 
 from caustics_amd.extended_source import mag_extended_source
@@ -137,6 +138,28 @@ def test_long_contours_match_oracle(gpu):
     mo = np.array([E.mag_extended_source(x, rho, nlenses=2, npts_limb=1500, limb_darkening=True,
                                          u1=0.5, s=0.8, q=0.5) for x in w])
     compare(m, mo, "binary LD npts_limb=1500")
+
+
+def test_chunks_and_streams_do_not_change_results(gpu):
+    # a batch cut into chunks that alternate over two streams, a ragged batch reordered
+    # by size, and one launch sequence on one stream give the same bits
+    from caustics_amd.extended_source import _run
+    from caustics_amd import _lib
+    g = np.random.default_rng(8)
+    S = 3000
+    w = torch.as_tensor(caustic_points(1e-2, npts=400, s=0.9, q=0.2)[:S], device="cuda")
+    S = w.numel()
+    rho = torch.as_tensor(10 ** g.uniform(-2.5, -1.5, S), device="cuda")
+    prm = torch.as_tensor(np.tile([0.9, 0.2], (S, 1)), device="cuda").contiguous()
+    u1 = torch.full((S,), 0.5, dtype=torch.float64, device="cuda")
+    gm = torch.as_tensor(g.normal(size=S), device="cuda")
+    npts = torch.as_tensor(g.choice([64, 128, 256, 600], S), dtype=torch.int32, device="cuda")
+    args = (2, w, rho, prm, npts, 600, True, u1, 40, 2500, _lib.MODE_FP64)
+    one = _run(*args, g=gm, streams=1, max_chunk=S)
+    many = _run(*args, g=gm, streams=2, max_chunk=257)
+    torch.cuda.synchronize()
+    for a, b in zip(one, many):
+        assert torch.equal(a, b)
 
 
 def test_ragged_batch_matches_individual(gpu):
