@@ -89,26 +89,29 @@ def _run(nlenses, w, rho_t, prm, npts_t, npts_max, ld, u1_t, npts_ld, itmax, mod
     for st in lanes:
         if st is not main:
             st.wait_stream(main)   # inputs and outputs were produced on the caller's stream
-    for ci, (lo, hi, nm) in enumerate(chunks):
-        sl = slice(lo, hi)
-        k = ci % nstr
-        with torch.cuda.stream(lanes[k]):
-            rc = L.caus_mag_extended_source_work(
-                _lib.stream_ptr(), nlenses, _lib.ptr(w[sl]), hi - lo, _lib.ptr(rho_t[sl]),
-                None if prm is None else _lib.ptr(prm[sl]), P,
-                None if npts_t is None else _lib.ptr(npts_t[sl]), nm, 1 if ld else 0,
-                None if u1_t is None else _lib.ptr(u1_t[sl]), 1, int(npts_ld),
-                None if g is None else _lib.ptr(g[sl]), _lib.ptr(mag[sl]),
-                None if g_out is None else _lib.ptr(g_out[sl]), _lib.ptr(status[sl]),
-                _lib.ptr(ws[k]), ws[k].numel() * 8, int(itmax), mode_id,
-                None if work is None else _lib.ptr(work))
-        _lib.check(rc, "caus_mag_extended_source")
-    for st in lanes:
-        if st is not main:
-            main.wait_stream(st)   # every later use of the outputs and workspaces is ordered
-            for t in [*ws, mag, status, g_out, w, rho_t, prm, npts_t, u1_t, g, work]:
-                if t is not None:
-                    t.record_stream(st)   # no reuse of their memory before the side work ends
+    try:
+        for ci, (lo, hi, nm) in enumerate(chunks):
+            sl = slice(lo, hi)
+            k = ci % nstr
+            with torch.cuda.stream(lanes[k]):
+                rc = L.caus_mag_extended_source_work(
+                    _lib.stream_ptr(), nlenses, _lib.ptr(w[sl]), hi - lo, _lib.ptr(rho_t[sl]),
+                    None if prm is None else _lib.ptr(prm[sl]), P,
+                    None if npts_t is None else _lib.ptr(npts_t[sl]), nm, 1 if ld else 0,
+                    None if u1_t is None else _lib.ptr(u1_t[sl]), 1, int(npts_ld),
+                    None if g is None else _lib.ptr(g[sl]), _lib.ptr(mag[sl]),
+                    None if g_out is None else _lib.ptr(g_out[sl]), _lib.ptr(status[sl]),
+                    _lib.ptr(ws[k]), ws[k].numel() * 8, int(itmax), mode_id,
+                    None if work is None else _lib.ptr(work))
+            _lib.check(rc, "caus_mag_extended_source")
+    finally:
+        # also after a failed launch: what was enqueued still orders every later use
+        for st in lanes:
+            if st is not main:
+                main.wait_stream(st)
+                for t in [*ws, mag, status, g_out, w, rho_t, prm, npts_t, u1_t, g, work]:
+                    if t is not None:
+                        t.record_stream(st)   # no reuse of their memory before the work ends
     if order is not None:
         back = lambda x: None if x is None else torch.empty_like(x).index_copy_(0, order, x)
         mag, status, g_out = back(mag), back(status), back(g_out)
