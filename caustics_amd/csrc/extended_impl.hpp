@@ -53,6 +53,7 @@ __host__ __device__ inline size_t ext_ws_bytes(int deg, int mmax) {
     b += ext_align((size_t)(mmax + 1) * sizeof(uint64_t));        // permutations / changepoints
     b += ext_align(2 * ((size_t)mmax * deg + 2) * sizeof(double)); // P, Q
     b += ext_align(2 * ((size_t)mmax * deg + 2) * sizeof(double)); // point cotangents
+    b += ext_align(2 * ((size_t)mmax * deg + 2) * sizeof(double)); // centroid cotangent parts
     b += ext_align(2 * (size_t)mmax * sizeof(int32_t));             // limb order x2
     b += ext_align(2 * (size_t)mmax * sizeof(uint64_t));            // gap keys x2
     return b;
@@ -65,6 +66,7 @@ struct ExtWS {
     uint64_t* perm;
     double* pq;
     double* xy;
+    double* yz;
     int32_t* ord[2];   // refinement: limb order -> index of the point in append order
     uint64_t* gk[2];   // refinement: key of the gap after each point in limb order
 };
@@ -86,6 +88,8 @@ __device__ inline ExtWS ext_ws(char* base, int deg, int mmax) {
     w.pq = (double*)(base + o);
     o += ext_align(2 * ((size_t)mmax * deg + 2) * sizeof(double));
     w.xy = (double*)(base + o);
+    o += ext_align(2 * ((size_t)mmax * deg + 2) * sizeof(double));
+    w.yz = (double*)(base + o);
     o += ext_align(2 * ((size_t)mmax * deg + 2) * sizeof(double));
     w.ord[0] = (int32_t*)(base + o);
     w.ord[1] = w.ord[0] + mmax;
@@ -530,52 +534,21 @@ __device__ __forceinline__ double ld_line(bool isP, double fixed, double a, doub
     return isP ? -0.5 * I : 0.5 * I;
 }
 
-// Contour integral over points c_0..c_{L-1} (pieces), closed back to c_0.
-// nterms trapezoid terms (L, or L-1 for the single-lens quirk); `cdiv` divides the
-// centroid sum (which includes the closing point) for limb darkening.
-template <int NL, int DEG, bool LD>
+// Uniform-source contour integral over points c_0..c_{L-1} (pieces), closed back to
+// c_0: nterms trapezoid terms of (x dy - y dx)/2 (L, or L-1 for the single-lens quirk).
+template <int NL, int DEG>
 __device__ double integrate_contour(const cd* zb, const Piece* pc, int np, int L, int nterms,
-                                    double cdiv, double* pq, const LensFrame<NL>& lf, cd w0,
-                                    double rho, double u1, const GLNodes& g, double* shd) {
+                                    double* shd) {
     const int tid = threadIdx.x;
     double acc = 0.0;
-    if (L <= 0) {
-        return block_sum(0.0, shd);
+    if (L <= 0) return block_sum(0.0, shd);
+    for (int k = tid; k < nterms; k += kExtBlock) {
+        const cd a = contour_point<DEG>(zb, pc, np, k);
+        const cd b = contour_point<DEG>(zb, pc, np, (k + 1 == L) ? 0 : k + 1);
+        acc += 0.5 * (0.5 * a.re + 0.5 * b.re) * (b.im - a.im) +
+               0.5 * (-0.5 * a.im - 0.5 * b.im) * (b.re - a.re);
     }
-    if constexpr (!LD) {
-        for (int k = tid; k < nterms; k += kExtBlock) {
-            const cd a = contour_point<DEG>(zb, pc, np, k);
-            const cd b = contour_point<DEG>(zb, pc, np, (k + 1 == L) ? 0 : k + 1);
-            acc += 0.5 * (0.5 * a.re + 0.5 * b.re) * (b.im - a.im) +
-                   0.5 * (-0.5 * a.im - 0.5 * b.im) * (b.re - a.re);
-        }
-        return block_sum(acc, shd);
-    } else {
-        double sx = 0.0, sy = 0.0;
-        for (int k = tid; k < L; k += kExtBlock) {
-            const cd a = contour_point<DEG>(zb, pc, np, k);
-            sx += a.re;
-            sy += a.im;
-        }
-        const cd c0 = contour_point<DEG>(zb, pc, np, 0);
-        const double zx = (block_sum(sx, shd) + c0.re) / cdiv;
-        const double zy = (block_sum(sy, shd) + c0.im) / cdiv;
-        for (int k = tid; k < L; k += kExtBlock) {
-            const cd a = contour_point<DEG>(zb, pc, np, k);
-            pq[2 * k] = ld_line<NL>(true, a.re, zy, a.im, rho, lf, w0, u1, g);
-            pq[2 * k + 1] = ld_line<NL>(false, a.im, zx, a.re, rho, lf, w0, u1, g);
-        }
-        __syncthreads();
-        for (int k = tid; k < nterms; k += kExtBlock) {
-            const int k1 = (k + 1 == L) ? 0 : k + 1;
-            const cd a = contour_point<DEG>(zb, pc, np, k);
-            const cd b = contour_point<DEG>(zb, pc, np, k1);
-            acc = ld_trapezoid(pq[2 * k], pq[2 * k1], pq[2 * k + 1], pq[2 * k1 + 1], a, b, acc);
-        }
-        const double r = block_sum(acc, shd);
-        __syncthreads();
-        return r;
-    }
+    return block_sum(acc, shd);
 }
 
 // ---- stitching (extended_source.py:312-667), wave-uniform code on wave 0 ---------
@@ -884,185 +857,163 @@ __device__ inline void image_adjoint(cd z, double th, const LensFrame<NL>& lf, d
     }
 }
 
-// Adjoint of one contour integral: adds scale * dI/d(frame, w0, rho, u1) to adj (per
-// thread).  pq holds P, Q of the contour points (limb darkening; written by the forward
-// pass).  Pass 1 forms each point's cotangent (and the direct dependence of its P, Q
-// lines), stored in xy; the centroid's cotangent is known only after a block reduction,
-// so pass 2 adds it and pushes every point's cotangent through the image tangents.
-template <int NL, int DEG, bool LD>
+// Adjoint of one uniform-source contour integral: adds scale * dI/d(frame, w0, rho) to
+// adj (per thread).  Each point's cotangent, from the two trapezoid terms it enters, is
+// pushed through the image tangents.
+template <int NL, int DEG>
 __device__ void contour_adjoint(const cd* zb, const double* theta, const Piece* pc, int np, int L,
-                                int nterms, double cdiv, const double* pq, double* xy,
-                                const LensFrame<NL>& lf, cd w0, double rho, double u1,
-                                const GLNodes& gl, double scale, LDAdj<NL>& adj, double* shd) {
+                                int nterms, const LensFrame<NL>& lf, double scale,
+                                LDAdj<NL>& adj) {
     const int tid = threadIdx.x;
-    if (L <= 0) return;
-    double zx = 0.0, zy = 0.0;
-    if constexpr (LD) {
-        double sx = 0.0, sy = 0.0;
-        for (int k = tid; k < L; k += kExtBlock) {
-            const cd a = contour_point<DEG>(zb, pc, np, k);
-            sx += a.re;
-            sy += a.im;
-        }
-        const cd c0 = contour_point<DEG>(zb, pc, np, 0);
-        zx = (block_sum(sx, shd) + c0.re) / cdiv;
-        zy = (block_sum(sy, shd) + c0.im) / cdiv;
-    }
-    double x0bar = 0.0, y0bar = 0.0;   // adjoints of the centroid (per thread)
     for (int k = tid; k < L; k += kExtBlock) {
-        const cd zk = contour_point<DEG>(zb, pc, np, k);
+        int idx, row;
+        const cd zk = contour_locate<DEG>(zb, pc, np, k, idx, row);
         const int kn = (k + 1 == L) ? 0 : k + 1;
         const int kp = (k == 0) ? L - 1 : k - 1;
-        const bool t_first = k < nterms;    // term k: (k, kn)
-        const bool t_second = kp < nterms;  // term kp: (kp, k)
         const cd zn = contour_point<DEG>(zb, pc, np, kn);
         const cd zp = contour_point<DEG>(zb, pc, np, kp);
         double xbar = 0.0, ybar = 0.0;
-        if constexpr (!LD) {
-            if (t_first) {
-                xbar += 0.5 * zn.im;
-                ybar -= 0.5 * zn.re;
-            }
-            if (t_second) {
-                xbar -= 0.5 * zp.im;
-                ybar += 0.5 * zp.re;
-            }
-        } else {
-            const double Pk = pq[2 * k], Qk = pq[2 * k + 1];
-            const double Pn = pq[2 * kn], Qn = pq[2 * kn + 1];
-            const double Pp = pq[2 * kp], Qp = pq[2 * kp + 1];
-            double Pbar = 0.0, Qbar = 0.0;
-            if (t_first) {
-                Pbar += 0.5 * (zn.re - zk.re);
-                Qbar += 0.5 * (zn.im - zk.im);
-                xbar -= 0.5 * (Pk + Pn);
-                ybar -= 0.5 * (Qk + Qn);
-            }
-            if (t_second) {
-                Pbar += 0.5 * (zk.re - zp.re);
-                Qbar += 0.5 * (zk.im - zp.im);
-                xbar += 0.5 * (Pp + Pk);
-                ybar += 0.5 * (Qp + Qk);
-            }
-            double dfix, da, db;
-            // P_k = P(x_k; y0 .. y_k)
-            ld_line_adj<NL>(true, zk.re, zy, zk.im, rho, lf, w0, u1, gl, scale * Pbar, dfix, da,
-                            db, adj);
-            xbar += Pbar * dfix;
-            ybar += Pbar * db;
-            y0bar += Pbar * da;
-            // Q_k = Q(y_k; x0 .. x_k)
-            ld_line_adj<NL>(false, zk.im, zx, zk.re, rho, lf, w0, u1, gl, scale * Qbar, dfix, da,
-                            db, adj);
-            ybar += Qbar * dfix;
-            xbar += Qbar * db;
-            x0bar += Qbar * da;
+        if (k < nterms) {
+            xbar += 0.5 * zn.im;
+            ybar -= 0.5 * zn.re;
         }
-        xy[2 * k] = xbar;
-        xy[2 * k + 1] = ybar;
+        if (kp < nterms) {
+            xbar -= 0.5 * zp.im;
+            ybar += 0.5 * zp.re;
+        }
+        image_adjoint<NL>(zk, theta[idx], lf, scale * xbar, scale * ybar, adj);
     }
-    double X0 = 0.0, Y0 = 0.0;
-    if constexpr (LD) {
-        X0 = block_sum(x0bar, shd) / cdiv;
-        Y0 = block_sum(y0bar, shd) / cdiv;
-    }
-    __syncthreads();
-    for (int k = tid; k < L; k += kExtBlock) {
-        int idx, row;
-        const cd zk = contour_locate<DEG>(zb, pc, np, k, idx, row);
-        const double wgt = (k == 0) ? 2.0 : 1.0;  // the closing point repeats c_0 in the centroid
-        image_adjoint<NL>(zk, theta[idx], lf, scale * (xy[2 * k] + wgt * X0),
-                          scale * (xy[2 * k + 1] + wgt * Y0), adj);
-    }
-    __syncthreads();
 }
 
-// Limb-darkened contour integral and its adjoint in one sweep over the Gauss-Legendre
-// nodes (forward + VJP): pass A evaluates every point's P, Q lines with their derivatives
-// and the direct parameter adjoint (the line cotangents Pbar, Qbar depend on the point
-// geometry only); pass B forms the trapezoid sum and the points' cotangents; pass C
-// pushes those through the image tangents.  Returns the integral, bit-identical to
-// integrate_contour<..., LD=true>.
-template <int NL, int DEG>
-__device__ double contour_ld_fused(const cd* zb, const double* theta, const Piece* pc, int np,
-                                   int L, int nterms, double cdiv, double* pq, double* xy,
-                                   const LensFrame<NL>& lf, cd w0, double rho, double u1,
-                                   const GLNodes& gl, double scale, LDAdj<NL>& adj, double* shd) {
+// One contour of a source as the integrals see it: its pieces, L points, nterms
+// trapezoid terms, the divisor of its centroid sum, its parity, the offset of its
+// points in the source's flat point list and (limb darkening) its centroid.
+struct ExtCEntry {
+    const Piece* pc;
+    int np, L, nterms, off;
+    double cdiv, par, zx, zy;
+};
+constexpr int kExtMaxContours = 16;  // >= nlenses^2 + 1 closed rows + 3 stitched contours
+
+// Limb-darkened integrals of all nc contours of a source (integrate.py:47-121), their
+// parity-weighted sum, and with VJP the reverse mode (adds to adj).  The P and Q line
+// integrals of every point -- 2 npts_ld brightness evaluations per point, nearly all of
+// the work -- run as one flat loop over the source's T points, so the workgroup idles only
+// in its last iteration; the centroids before it and the trapezoid sums and the point
+// adjoints after it run contour by contour, and every sum rounds as it would in a
+// contour-by-contour evaluation.  The forward and forward+VJP kernels form P and Q
+// alike (ld_line / ld_line_adj), so they return the same integral.
+// pq, xy, yz hold 2 T doubles each: P and Q, the points' cotangents, and the parts of the
+// centroid's cotangent (Pbar dP/dy0, Qbar dQ/dx0).
+template <int NL, int DEG, bool VJP>
+__device__ double integrate_source_ld(const cd* zb, const double* theta, ExtCEntry* ct, int nc,
+                                      int T, double* pq, double* xy, double* yz,
+                                      const LensFrame<NL>& lf, cd w0, double rho, double u1,
+                                      const GLNodes& gl, LDAdj<NL>& adj, double* shd) {
     const int tid = threadIdx.x;
-    if (L <= 0) return block_sum(0.0, shd);
-    double sx = 0.0, sy = 0.0;
-    for (int k = tid; k < L; k += kExtBlock) {
-        const cd a = contour_point<DEG>(zb, pc, np, k);
-        sx += a.re;
-        sy += a.im;
-    }
-    const cd c0 = contour_point<DEG>(zb, pc, np, 0);
-    const double zx = (block_sum(sx, shd) + c0.re) / cdiv;
-    const double zy = (block_sum(sy, shd) + c0.im) / cdiv;
-    double x0bar = 0.0, y0bar = 0.0;
-    // pass A
-    for (int k = tid; k < L; k += kExtBlock) {
-        const cd zk = contour_point<DEG>(zb, pc, np, k);
-        const int kn = (k + 1 == L) ? 0 : k + 1;
-        const int kp = (k == 0) ? L - 1 : k - 1;
-        const cd zn = contour_point<DEG>(zb, pc, np, kn);
-        const cd zp = contour_point<DEG>(zb, pc, np, kp);
-        double Pbar = 0.0, Qbar = 0.0;
-        if (k < nterms) {
-            Pbar += 0.5 * (zn.re - zk.re);
-            Qbar += 0.5 * (zn.im - zk.im);
+    // centroids (integrate.py:112); the closing point repeats c_0 in the sum
+    for (int c = 0; c < nc; ++c) {
+        const ExtCEntry& e = ct[c];
+        double sx = 0.0, sy = 0.0;
+        for (int k = tid; k < e.L; k += kExtBlock) {
+            const cd a = contour_point<DEG>(zb, e.pc, e.np, k);
+            sx += a.re;
+            sy += a.im;
         }
-        if (kp < nterms) {
-            Pbar += 0.5 * (zk.re - zp.re);
-            Qbar += 0.5 * (zk.im - zp.im);
+        const cd c0 = (e.L > 0) ? contour_point<DEG>(zb, e.pc, e.np, 0) : cd{0.0, 0.0};
+        const double zx = (block_sum(sx, shd) + c0.re) / e.cdiv;
+        const double zy = (block_sum(sy, shd) + c0.im) / e.cdiv;
+        if (tid == 0) {
+            ct[c].zx = zx;
+            ct[c].zy = zy;
         }
-        double dfP, daP, dbP, dfQ, daQ, dbQ;
-        pq[2 * k] = ld_line_adj<NL>(true, zk.re, zy, zk.im, rho, lf, w0, u1, gl, scale * Pbar,
-                                    dfP, daP, dbP, adj);
-        pq[2 * k + 1] = ld_line_adj<NL>(false, zk.im, zx, zk.re, rho, lf, w0, u1, gl,
-                                        scale * Qbar, dfQ, daQ, dbQ, adj);
-        xy[2 * k] = Pbar * dfP + Qbar * dbQ;
-        xy[2 * k + 1] = Pbar * dbP + Qbar * dfQ;
-        y0bar += Pbar * daP;
-        x0bar += Qbar * daQ;
     }
     __syncthreads();
-    // pass B
-    double acc = 0.0;
-    for (int k = tid; k < L; k += kExtBlock) {
-        const int kn = (k + 1 == L) ? 0 : k + 1;
-        const int kp = (k == 0) ? L - 1 : k - 1;
-        const double Pk = pq[2 * k], Qk = pq[2 * k + 1];
-        const double Pn = pq[2 * kn], Qn = pq[2 * kn + 1];
-        const double Pp = pq[2 * kp], Qp = pq[2 * kp + 1];
-        double xbar = xy[2 * k], ybar = xy[2 * k + 1];
-        if (k < nterms) {
-            const cd a = contour_point<DEG>(zb, pc, np, k);
-            const cd b = contour_point<DEG>(zb, pc, np, kn);
-            acc = ld_trapezoid(Pk, Pn, Qk, Qn, a, b, acc);
-            xbar -= 0.5 * (Pk + Pn);
-            ybar -= 0.5 * (Qk + Qn);
+    // the P and Q lines of every point, flat over the contours
+    for (int gi = tid; gi < T; gi += kExtBlock) {
+        int c = 0;
+        while (c + 1 < nc && gi >= ct[c + 1].off) ++c;
+        const ExtCEntry& e = ct[c];
+        const int k = gi - e.off;
+        const cd zk = contour_point<DEG>(zb, e.pc, e.np, k);
+        if constexpr (!VJP) {
+            pq[2 * gi] = ld_line<NL>(true, zk.re, e.zy, zk.im, rho, lf, w0, u1, gl);
+            pq[2 * gi + 1] = ld_line<NL>(false, zk.im, e.zx, zk.re, rho, lf, w0, u1, gl);
+        } else {
+            const int L = e.L;
+            const int kn = (k + 1 == L) ? 0 : k + 1;
+            const int kp = (k == 0) ? L - 1 : k - 1;
+            const cd zn = contour_point<DEG>(zb, e.pc, e.np, kn);
+            const cd zp = contour_point<DEG>(zb, e.pc, e.np, kp);
+            // cotangents of P_k, Q_k from the two trapezoid terms they enter
+            double Pbar = 0.0, Qbar = 0.0;
+            if (k < e.nterms) {
+                Pbar += 0.5 * (zn.re - zk.re);
+                Qbar += 0.5 * (zn.im - zk.im);
+            }
+            if (kp < e.nterms) {
+                Pbar += 0.5 * (zk.re - zp.re);
+                Qbar += 0.5 * (zk.im - zp.im);
+            }
+            double dfP, daP, dbP, dfQ, daQ, dbQ;
+            pq[2 * gi] = ld_line_adj<NL>(true, zk.re, e.zy, zk.im, rho, lf, w0, u1, gl,
+                                         e.par * Pbar, dfP, daP, dbP, adj);
+            pq[2 * gi + 1] = ld_line_adj<NL>(false, zk.im, e.zx, zk.re, rho, lf, w0, u1, gl,
+                                             e.par * Qbar, dfQ, daQ, dbQ, adj);
+            xy[2 * gi] = Pbar * dfP + Qbar * dbQ;
+            xy[2 * gi + 1] = Pbar * dbP + Qbar * dfQ;
+            yz[2 * gi] = Pbar * daP;       // P_k = P(x_k; y0 .. y_k): to the centroid's y
+            yz[2 * gi + 1] = Qbar * daQ;   // Q_k = Q(y_k; x0 .. x_k): to the centroid's x
         }
-        if (kp < nterms) {
-            xbar += 0.5 * (Pp + Pk);
-            ybar += 0.5 * (Qp + Qk);
-        }
-        xy[2 * k] = xbar;
-        xy[2 * k + 1] = ybar;
-    }
-    const double I = block_sum(acc, shd);
-    const double X0 = block_sum(x0bar, shd) / cdiv;
-    const double Y0 = block_sum(y0bar, shd) / cdiv;
-    // pass C
-    for (int k = tid; k < L; k += kExtBlock) {
-        int idx, row;
-        const cd zk = contour_locate<DEG>(zb, pc, np, k, idx, row);
-        const double wgt = (k == 0) ? 2.0 : 1.0;  // the closing point repeats c_0 in the centroid
-        image_adjoint<NL>(zk, theta[idx], lf, scale * (xy[2 * k] + wgt * X0),
-                          scale * (xy[2 * k + 1] + wgt * Y0), adj);
     }
     __syncthreads();
-    return I;
+    double total = 0.0;
+    for (int c = 0; c < nc; ++c) {
+        const ExtCEntry& e = ct[c];
+        const int L = e.L;
+        const double* q = pq + 2 * e.off;
+        double* v = xy + 2 * e.off;
+        const double* y = yz + 2 * e.off;
+        double acc = 0.0, x0bar = 0.0, y0bar = 0.0;
+        for (int k = tid; k < L; k += kExtBlock) {
+            const int kn = (k + 1 == L) ? 0 : k + 1;
+            if (k < e.nterms) {
+                const cd a = contour_point<DEG>(zb, e.pc, e.np, k);
+                const cd b = contour_point<DEG>(zb, e.pc, e.np, kn);
+                acc = ld_trapezoid(q[2 * k], q[2 * kn], q[2 * k + 1], q[2 * kn + 1], a, b, acc);
+            }
+            if constexpr (VJP) {
+                const int kp = (k == 0) ? L - 1 : k - 1;
+                double xbar = v[2 * k], ybar = v[2 * k + 1];
+                if (k < e.nterms) {
+                    xbar -= 0.5 * (q[2 * k] + q[2 * kn]);
+                    ybar -= 0.5 * (q[2 * k + 1] + q[2 * kn + 1]);
+                }
+                if (kp < e.nterms) {
+                    xbar += 0.5 * (q[2 * kp] + q[2 * k]);
+                    ybar += 0.5 * (q[2 * kp + 1] + q[2 * k + 1]);
+                }
+                v[2 * k] = xbar;
+                v[2 * k + 1] = ybar;
+                y0bar += y[2 * k];
+                x0bar += y[2 * k + 1];
+            }
+        }
+        total += e.par * block_sum(acc, shd);
+        if constexpr (VJP) {
+            const double X0 = block_sum(x0bar, shd) / e.cdiv;
+            const double Y0 = block_sum(y0bar, shd) / e.cdiv;
+            // each thread reads back only the cotangents it wrote above
+            for (int k = tid; k < L; k += kExtBlock) {
+                int idx, row;
+                const cd zk = contour_locate<DEG>(zb, e.pc, e.np, k, idx, row);
+                const double wgt = (k == 0) ? 2.0 : 1.0;  // c_0 enters the centroid twice
+                image_adjoint<NL>(zk, theta[idx], lf, e.par * (v[2 * k] + wgt * X0),
+                                  e.par * (v[2 * k + 1] + wgt * Y0), adj);
+            }
+        }
+    }
+    return total;
 }
 
 }  // namespace caus
@@ -1089,7 +1040,7 @@ namespace caus {
 // waves per SIMD that the register allocation of the reverse-mode integrate kernel
 // targets (its per-thread adjoint state is large)
 #ifndef CAUS_EXT_VJP_WAVES
-#define CAUS_EXT_VJP_WAVES 2
+#define CAUS_EXT_VJP_WAVES 3
 #endif
 
 constexpr int kExtPer = 4;          // initial limb angles per lane of k_ext_limb_init
@@ -1829,53 +1780,64 @@ void k_ext_integrate(ExtArgs A, int buf) {
     const cd* zb = W.z[buf];
     const uint32_t* fb = W.flg[buf];
     const double* thb = W.theta[buf];
-    double total = 0.0;
+    // the source's contours (extended_source.py:860-899): NL == 1 both images (tail index
+    // N-1, :860-866); otherwise every closed image row, then the three stitched contours
+    __shared__ ExtCEntry s_ct[kExtMaxContours];
+    __shared__ Piece s_one[DEG];
+    __shared__ int s_nc, s_T;
+    if (tid == 0) {
+        int nc = 0, off = 0;
+        auto add = [&](const Piece* pc, int np, int L, int nterms, double cdiv, double par) {
+            s_ct[nc] = ExtCEntry{pc, np, L, nterms, off, cdiv, par, 0.0, 0.0};
+            off += L;
+            ++nc;
+        };
+        if constexpr (NL == 1) {
+            for (int r = 0; r < 2; ++r) {
+                s_one[r] = Piece{r, 0, M, 0};
+                add(&s_one[r], 1, M, M - 1, (double)M, ((fb[0] >> (16 + r)) & 1u) ? -1.0 : 1.0);
+            }
+        } else {
+            const ExtContours* C = ext_contours<DEG>(A, src);
+            const unsigned closed = C->closed;
+            for (int r = 0; r < DEG; ++r) {
+                if ((closed >> r) & 1u) {
+                    s_one[r] = Piece{r, 0, M, 0};
+                    add(&s_one[r], 1, M, M, (double)(M + 1),
+                        ((fb[0] >> (16 + r)) & 1u) ? -1.0 : 1.0);
+                }
+            }
+            if (C->open) {
+                for (int ci = 0; ci < 3; ++ci) {
+                    const ActSeg& a = C->con[ci];
+                    const int L = a.empty ? 0 : a.t + 1;
+                    add(a.pc, a.np, L, L, (double)(L + 1), (double)a.par);
+                }
+            }
+        }
+        s_nc = nc;
+        s_T = off;
+    }
+    __syncthreads();
+    const int nc = s_nc, T = s_T;
     double gacc[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) gacc[d] = 0.0;
     LDAdj<NL> adj;
     adj.zero();
-    unsigned long long npoints = 0;
-    auto do_contour = [&](const Piece* pcs, int npc, int L, int nterms, double cdiv, int par) {
-        npoints += (L > 0) ? (unsigned long long)L : 0ull;
-        if constexpr (LD && VJP) {
-            total += par * contour_ld_fused<NL, DEG>(zb, thb, pcs, npc, L, nterms, cdiv, W.pq,
-                                                     W.xy, lf, w0, rho, u1, g, (double)par, adj,
-                                                     s_d);
-        } else {
-            const double I = integrate_contour<NL, DEG, LD>(zb, pcs, npc, L, nterms, cdiv, W.pq,
-                                                            lf, w0, rho, u1, g, s_d);
-            total += par * I;
-            if constexpr (VJP)
-                contour_adjoint<NL, DEG, LD>(zb, thb, pcs, npc, L, nterms, cdiv, W.pq, W.xy, lf,
-                                             w0, rho, u1, g, (double)par, adj, s_d);
-        }
-    };
-    if constexpr (NL == 1) {
-        // both images, tail index N-1 (extended_source.py:860-866)
-        for (int r = 0; r < 2; ++r) {
-            Piece pc{r, 0, M, 0};
-            const int par = ((fb[0] >> (16 + r)) & 1u) ? -1 : 1;
-            do_contour(&pc, 1, M, M - 1, (double)M, par);
-        }
+    double total = 0.0;
+    if constexpr (LD) {
+        total = integrate_source_ld<NL, DEG, VJP>(zb, thb, s_ct, nc, T, W.pq, W.xy, W.yz, lf, w0,
+                                                  rho, u1, g, adj, s_d);
     } else {
-        const ExtContours* C = ext_contours<DEG>(A, src);
-        const unsigned closed = C->closed;
-        for (int r = 0; r < DEG; ++r) {
-            if ((closed >> r) & 1u) {
-                Piece pc{r, 0, M, 0};
-                const int par = ((fb[0] >> (16 + r)) & 1u) ? -1 : 1;
-                do_contour(&pc, 1, M, M, (double)(M + 1), par);
-            }
-        }
-        if (C->open) {
-            for (int ci = 0; ci < 3; ++ci) {
-                const ActSeg& a = C->con[ci];
-                const int L = a.empty ? 0 : a.t + 1;
-                do_contour(a.pc, a.np, L, L, (double)(L + 1), a.par);
-            }
+        for (int c = 0; c < nc; ++c) {
+            const ExtCEntry& e = s_ct[c];
+            total += e.par * integrate_contour<NL, DEG>(zb, e.pc, e.np, e.L, e.nterms, s_d);
+            if constexpr (VJP)
+                contour_adjoint<NL, DEG>(zb, thb, e.pc, e.np, e.L, e.nterms, lf, e.par, adj);
         }
     }
+    const unsigned long long npoints = (unsigned long long)T;
     const double m = fabs(total) / (kExtPi * rho * rho);
     if constexpr (VJP) {
         const double base = (total >= 0.0 ? 1.0 : -1.0) / (kExtPi * rho * rho);
