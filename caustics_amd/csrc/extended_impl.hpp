@@ -1226,7 +1226,9 @@ static __global__ __launch_bounds__(kExtPlanBlock) void k_ext_plan(ExtArgs A) {
 // image is found twice and its partner never; independent solves keep the image sets
 // of the oracle.
 template <int NL, int MODE>
-__global__ __launch_bounds__(kExtFlatBlock) void k_ext_limb_init(ExtArgs A) {
+__global__ __launch_bounds__(kExtFlatBlock)
+__attribute__((amdgpu_waves_per_eu(2)))  // 2 waves/SIMD measured faster than 1
+void k_ext_limb_init(ExtArgs A) {
     constexpr int DEG = ext_degree<NL>();
     const ExtBatch B = ext_batch(A);
     const int64_t total = B.off_init[A.S];
@@ -1403,7 +1405,9 @@ __global__ __launch_bounds__(NT) void k_ext_select(ExtArgs A, int cur, int round
 // solved, matched to its left neighbour's images (match_points), appended, and the
 // keys of its two gaps
 template <int NL, int MODE>
-__global__ __launch_bounds__(kExtFlatBlock) void k_ext_solve_add(ExtArgs A, int cur) {
+__global__ __launch_bounds__(kExtFlatBlock)
+__attribute__((amdgpu_waves_per_eu(2)))  // 2 waves/SIMD measured faster than 1
+void k_ext_solve_add(ExtArgs A, int cur) {
     constexpr int DEG = ext_degree<NL>();
     const ExtBatch B = ext_batch(A);
     const int64_t total = B.off_add[A.S];
