@@ -9,7 +9,9 @@ from .extended_source import mag_extended_source
 from .lightcurve import (hexadecapole_gate, hexadecapole_vjp, mag, mag_fwd_vjp,
                          mag_point_source_lightcurve)
 from .linalg import marginalized_log_likelihood
-from .point_source import critical_and_caustic_curves, lens_eq, lens_eq_det_jac, mag_point_source
+from .point_source import (critical_and_caustic_curves, images_point_source,
+                           images_point_source_sequential, lens_eq, lens_eq_det_jac,
+                           mag_point_source)
 
 __all__ = [
     "ehrlich_aberth",
@@ -17,6 +19,8 @@ __all__ = [
     "lens_eq",
     "lens_eq_det_jac",
     "mag_point_source",
+    "images_point_source",
+    "images_point_source_sequential",
     "critical_and_caustic_curves",
     "mag_point_source_lightcurve",
     "mag_extended_source",

@@ -54,6 +54,8 @@ SIGNATURES = {
                                                      _i32, _i32, _vp, _i64, _i32, _vp, _vp, _vp,
                                                      _vp, _vp, _i64, _i32, _i32, _vp]),
     "caus_match_chain": (ctypes.c_int, [_vp, _vp, _vp, _i64, _i32, _i64]),
+    "caus_images_point_source": (ctypes.c_int, [_vp, _i32, _vp, _i64, _i64, _vp, _i64, _vp, _vp,
+                                                _vp, _vp, _i32, _i32, _i32]),
     "caus_marginalized_log_likelihood": (ctypes.c_int, [_vp, _vp, _i64, _i64, _i64, _vp, _vp,
                                                         _vp, _i32, _vp, _vp, _vp, _vp, _i64]),
     "caus_hexadecapole_gate": (ctypes.c_int, [_vp, _i32, _vp, _i64, _vp, _i64, _vp, _i64,

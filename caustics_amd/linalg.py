@@ -17,6 +17,7 @@ the observations are data.  The reference's dense-covariance branch does not run
 ``dense_covariance=True`` raises.  ``Lam_sd`` belongs to that branch and is unused, as
 in the reference.
 """
+import numpy as np
 import torch
 
 from . import _lib
@@ -54,7 +55,9 @@ class _MargLL(torch.autograd.Function):
 
 
 def _f64(x, device):
-    return torch.as_tensor(x, device=device).to(torch.float64).contiguous()
+    if torch.is_tensor(x):
+        return x.to(device=device, dtype=torch.float64).contiguous()
+    return torch.as_tensor(np.asarray(x, np.float64), device=device).contiguous()
 
 
 def marginalized_log_likelihood(A_list, fobs_list, C_inv_list, dense_covariance=False,
@@ -66,7 +69,8 @@ def marginalized_log_likelihood(A_list, fobs_list, C_inv_list, dense_covariance=
     if not (len(A_list) == len(fobs_list) == len(C_inv_list)) or len(A_list) == 0:
         raise ValueError("A_list, fobs_list and C_inv_list must be non-empty and of equal length")
     dev = torch.device("cuda")
-    As = [a if isinstance(a, torch.Tensor) else torch.as_tensor(a) for a in A_list]
+    As = [a if isinstance(a, torch.Tensor) else torch.as_tensor(np.asarray(a, np.float64))
+          for a in A_list]
     batched = As[0].dim() == 2
     if any((a.dim() == 2) != batched for a in As):
         raise ValueError("all magnification arrays must be 1-D, or all (B, n_k)")

@@ -9,6 +9,13 @@ parameter through det J.
 
 ``lens_eq`` and ``lens_eq_det_jac`` (:1536-1580) take lens-frame parameters
 (a, e1[, e2, r3]) exactly as in the reference and are elementwise tensor expressions.
+
+``images_point_source`` and ``images_point_source_sequential`` (:1652-1759, the
+reference's private ``_images_point_source`` and ``_images_point_source_sequential``)
+return the images and the real-image mask for lens-frame source positions and
+lens-frame parameters; the sequential form warm-starts each position from the previous
+one's roots, so that row j follows one image along the sequence.  They return no
+gradients (``mag_point_source`` and ``ehrlich_aberth`` do).
 """
 import numpy as np
 import torch
@@ -17,6 +24,8 @@ from . import _lib
 
 __all__ = [
     "mag_point_source",
+    "images_point_source",
+    "images_point_source_sequential",
     "lens_eq",
     "lens_eq_det_jac",
     "critical_and_caustic_curves",
@@ -118,6 +127,107 @@ def mag_point_source(w, nlenses=2, roots_itmax=2500, roots_compensated=False, mo
                                  w, *ps)
 
 
+_DEGREE = {1: 2, 2: 5, 3: 10}
+
+
+def _as64(v):
+    """v on the device in 64-bit precision (float64, or complex128 when complex);
+    torch.as_tensor alone makes python floats and lists float32."""
+    if torch.is_tensor(v):
+        return v.to(device="cuda", dtype=torch.complex128 if v.is_complex() else torch.float64)
+    a = np.asarray(v)
+    return torch.as_tensor(a.astype(np.complex128 if np.iscomplexobj(a) else np.float64),
+                           device="cuda")
+
+
+def _lens_rows(nlenses, shape, params):
+    """Lens-frame parameter rows (a, e1) or (a, e1, e2, Re r3, Im r3) broadcast to
+    `shape` (the source positions' batch shape, sequences flattened)."""
+    if nlenses == 1:
+        return None
+    names = ("a", "e1") if nlenses == 2 else ("a", "e1", "e2", "r3")
+    cols = []
+    for n in names:
+        v = params[n]
+        if n == "r3":
+            # python floats and complexes straight to 64-bit (as_tensor would make float32)
+            v = torch.as_tensor(v, dtype=torch.complex128, device="cuda")
+            cols += [v.real, v.imag]
+        else:
+            v = (v.to(device="cuda", dtype=torch.float64) if torch.is_tensor(v)
+                 else torch.as_tensor(v, dtype=torch.float64, device="cuda"))
+            cols.append(v)
+    cols = [c.expand(shape).reshape(-1) for c in cols]
+    return torch.stack(cols, -1).contiguous()
+
+
+def _images(w, nlenses, itmax, mode, z_init, sequential, params):
+    L = _lib.lib()
+    n = _DEGREE[nlenses]
+    B, N = w.shape
+    lens = _lens_rows(nlenses, (B,) if sequential else (B, N), params)
+    if lens is not None and not sequential:
+        lens = lens.reshape(B * N, -1)
+    z = torch.empty((B, N, n), dtype=torch.complex128, device="cuda")
+    mask = torch.empty((B, N, n), dtype=torch.uint8, device="cuda")
+    status = torch.empty((B, N), dtype=torch.uint8, device="cuda")
+    zi = None
+    if z_init is not None:
+        zi = (z_init.to(device="cuda", dtype=torch.complex128) if torch.is_tensor(z_init)
+              else torch.as_tensor(np.asarray(z_init, np.complex128), device="cuda"))
+        zi = zi.reshape(n, B, N).permute(1, 2, 0).contiguous()
+    # independent positions: one lens row per position, addressed as B*N sequences of 1
+    Bc, Nc = (B, N) if sequential else (B * N, 1)
+    rc = L.caus_images_point_source(_lib.stream_ptr(), nlenses, _lib.ptr(w), Bc, Nc,
+                                    _lib.ptr(lens), 0 if lens is None else lens.shape[1],
+                                    _lib.ptr(zi), _lib.ptr(z), _lib.ptr(mask), _lib.ptr(status),
+                                    int(itmax), mode, 1 if sequential else 0)
+    _lib.check(rc, "caus_images_point_source")
+    return z, mask.bool(), status
+
+
+def images_point_source(w, nlenses=2, roots_itmax=2500, roots_compensated=False,
+                        custom_init=False, z_init=None, mode=None, **params):
+    """Images (n, *w.shape) complex and the real-image mask (|lens_eq(z) - w| < 1e-6) of
+    lens-frame source positions w; parameters a, e1 (nlenses=2) or a, e1, e2, r3
+    (nlenses=3), floats or tensors broadcastable against w.  custom_init with z_init
+    (n, *w.shape) starts the solver from those roots (point_source.py:1652-1709)."""
+    if nlenses not in (1, 2, 3):
+        raise ValueError("`nlenses` has to be set to be <= 3.")
+    w = (w.to(device="cuda", dtype=torch.complex128) if torch.is_tensor(w)
+         else torch.as_tensor(np.asarray(w, np.complex128), device="cuda"))
+    shape = w.shape
+    wf = w.reshape(1, -1).contiguous()
+    prm = {k: (_as64(v).expand(shape).reshape(1, -1) if torch.is_tensor(v) or np.ndim(v) else v)
+           for k, v in params.items()}
+    z, mask, _ = _images(wf, nlenses, roots_itmax, _mode_id(roots_compensated, mode),
+                         z_init if custom_init else None, False, prm)
+    n = _DEGREE[nlenses]
+    return (z.reshape(*shape, n).movedim(-1, 0), mask.reshape(*shape, n).movedim(-1, 0))
+
+
+def images_point_source_sequential(w, nlenses=2, roots_itmax=2500, roots_compensated=False,
+                                   mode=None, **params):
+    """Images along a sequence of lens-frame source positions w (N,) solved in order, each
+    position started from the previous one's roots (point_source.py:1711-1759): returns
+    z (n, N) and the mask (n, N).  A 2-D w (B, N) runs B sequences at once, with
+    parameters of shape (B,) or scalars, and returns (B, n, N)."""
+    if nlenses not in (1, 2, 3):
+        raise ValueError("`nlenses` has to be set to be <= 3.")
+    w = (w.to(device="cuda", dtype=torch.complex128) if torch.is_tensor(w)
+         else torch.as_tensor(np.asarray(w, np.complex128), device="cuda"))
+    one = w.dim() == 1
+    w2 = (w.reshape(1, -1) if one else w).contiguous()
+    B = w2.shape[0]
+    prm = {k: (_as64(v).reshape(-1) if torch.is_tensor(v) or np.ndim(v) else v)
+           for k, v in params.items()}
+    z, mask, _ = _images(w2, nlenses, roots_itmax, _mode_id(roots_compensated, mode), None,
+                         True, {k: (v if not torch.is_tensor(v) else v.expand(B)) for k, v in
+                                prm.items()})
+    z, mask = z.permute(0, 2, 1), mask.permute(0, 2, 1)   # (B, n, N)
+    return (z[0], mask[0]) if one else (z, mask)
+
+
 def _lens_terms(z, nlenses, params):
     zb = torch.conj(z)
     if nlenses == 1:
@@ -126,7 +236,8 @@ def _lens_terms(z, nlenses, params):
     if nlenses == 2:
         return [(e1, a), (1.0 - e1, -a)], zb
     e2, r3 = params["e2"], params["r3"]
-    r3 = r3 if isinstance(r3, torch.Tensor) else torch.as_tensor(complex(r3), device=z.device)
+    r3 = r3 if isinstance(r3, torch.Tensor) else torch.as_tensor(complex(r3), device=z.device,
+                                                                 dtype=torch.complex128)
     return [(e1, a), (e2, -a), (1.0 - e1 - e2, r3)], zb
 
 

@@ -205,6 +205,24 @@ int caus_mag_extended_source_work(void* stream, int32_t nlenses, const double* w
 int caus_match_chain(void* stream, const double* pts, double* out, int64_t npts, int32_t n,
                      int64_t nchains);
 
+/* Images of point sources in the lens frame (reference: _images_point_source,
+ * src/caustics/point_source.py:1652-1709) and their sequential, warm-started form
+ * (_images_point_source_sequential, :1711-1759).
+ *   w (B, N) complex source positions in the lens frame
+ *   lens (B rows, stride lens_stride doubles): (a, e1) for nlenses 2, (a, e1, e2, Re r3,
+ *     Im r3) for 3 (lens 1 at +a with mass e1, lens 2 at -a, lens 3 at r3); unused for 1
+ *   z_init (B, N, n) complex or NULL: initial roots (the reference's custom_init); with
+ *     sequential = 1 only point 0 of each sequence uses them
+ *   sequential 0: every position solved on its own; 1: each of the B sequences solved in
+ *     order, position k > 0 started from position k-1's roots, so that root j follows one
+ *     image along the sequence
+ *   z (B, N, n) complex out; mask (B, N, n) uint8 out, 1 where |lens_eq(z) - w| < 1e-6;
+ *   status (B, N) uint8 or NULL (1 = converged).  n = 2, 5, 10 for nlenses 1, 2, 3. */
+int caus_images_point_source(void* stream, int32_t nlenses, const double* w, int64_t B,
+                             int64_t N, const double* lens, int64_t lens_stride,
+                             const double* z_init, double* z, uint8_t* mask, uint8_t* status,
+                             int32_t itmax, int32_t mode, int32_t sequential);
+
 /* Flux-marginalised log-likelihood with a diagonal data covariance for B magnification
  * curves that share one set of observations (reference: marginalized_log_likelihood,
  * src/caustics/linalg.py:10-70, dense_covariance=False; the lists of the reference are

@@ -196,6 +196,29 @@ def images_point_source(w, nlenses=2, itmax=2500, compensated=False, init_mode=0
     return z, mask
 
 
+def images_point_source_sequential(w, nlenses=2, itmax=2500, compensated=False, init_mode=0,
+                                   **lp):
+    """(n, N) images and mask along the 1-D sequence w (point_source.py:1711-1759): the
+    first position from the default initial estimates, each later one with the previous
+    position's roots as initial roots (custom_init, ehrlich_aberth.h:89-94)."""
+    w = np.asarray(w, complex).reshape(-1)
+    if nlenses == 1:
+        return images_point_source(w, nlenses)
+    c = poly_coeffs(w, nlenses, **lp)              # (N, n+1) highest-first
+    n = c.shape[-1] - 1
+    z = np.zeros((w.size, n), complex)
+    prev = None
+    for k in range(w.size):
+        ck = c[k:k + 1, ::-1].copy()               # ascending
+        zk = ea.roots_asc(ck, itmax=itmax, compensated=compensated, roots_init=prev,
+                          init_mode=init_mode)
+        z[k] = zk.reshape(-1)
+        prev = z[k:k + 1]
+    z = z.T
+    mask = np.abs(lens_eq(z, nlenses, **lp) - w) < 1e-6
+    return z, mask
+
+
 def mag_point_source(w, nlenses=2, itmax=2500, compensated=False, init_mode=0, perturb=0.0,
                      seed=0, perturb_inputs=False, **params):
     """Point-source magnification (point_source.py:1762-1830).

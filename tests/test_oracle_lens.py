@@ -89,3 +89,19 @@ def test_lens_equation_route_agrees_with_polynomial_route():
             zp = zp[:, 0]
             near = zp[np.argmin(np.abs(zp[:, None] - z[None, :]), axis=0)]
             np.testing.assert_allclose((near - z) / h, dz, rtol=1e-5, atol=1e-6)
+
+
+def test_sequential_images_are_the_same_roots_followed_continuously():
+    # images_point_source_sequential (point_source.py:1711-1759): the same root set as
+    # independent solves at every position, with each row the continuation of itself
+    lp = dict(a=0.6, e1=0.5, e2=0.3, r3=0.2 + 0.7j)
+    t = np.linspace(-1.5, 1.5, 300)
+    w = (0.2j + t) * np.exp(0.3j)
+    zs, ms = lens.images_point_source_sequential(w, nlenses=3, init_mode=1, **lp)
+    zi, mi = lens.images_point_source(w, nlenses=3, init_mode=1, **lp)
+    d = np.abs(np.sort_complex(zs.T) - np.sort_complex(zi.T))
+    assert np.max(d) < 1e-9
+    step = np.abs(np.diff(zs, axis=1))
+    alt = np.abs(zs[None, :, 1:] - zs[:, None, :-1]).min(axis=1)
+    assert np.mean(step <= alt + 1e-12) > 0.99
+    assert ms.sum() == mi.sum()
