@@ -167,10 +167,15 @@ __device__ __forceinline__ uint64_t nn_map(const cd (&a)[DEG], const cd (&b)[DEG
 }
 
 // ---- point solve -----------------------------------------------------------
+// warm = true: z holds initial roots (those of a nearby limb point).  The solve ends
+// with the compensated polish, which takes converged roots to the doubles nearest the
+// exact roots, so the result does not depend on where the iteration started (except for
+// clustered roots) and the rows are put in order afterwards by value (match_points).
 template <int NL, int MODE>
 __device__ __forceinline__ bool ext_solve_point(cd w, const LensFrame<NL>& lf,
                                                 cd (&z)[ext_degree<NL>()], uint32_t& flags,
-                                                int itmax, SolveStats* stats = nullptr) {
+                                                int itmax, SolveStats* stats = nullptr,
+                                                bool warm = false) {
     constexpr int DEG = ext_degree<NL>();
     bool ok = true;
     flags = 0;
@@ -185,7 +190,7 @@ __device__ __forceinline__ bool ext_solve_point(cd w, const LensFrame<NL>& lf,
             if (norm2(z[j]) < 1.0) flags |= 1u << (16 + j);
         }
     } else {
-        ok = solve_lens<NL, MODE, true>(w, lf, z, itmax, false, stats);
+        ok = solve_lens<NL, MODE, true>(w, lf, z, itmax, warm, stats);
 #pragma unroll
         for (int j = 0; j < DEG; ++j) {
             ImageTerms<NL> it = image_terms<NL>(z[j], lf);
@@ -1255,8 +1260,9 @@ void k_ext_limb_init(ExtArgs A) {
             double sn, cs;
             sincos(th, &sn, &cs);
             uint32_t f;
+            // the lane's later angles start from the previous angle's roots
             if (!ext_solve_point<NL, MODE>(w0 + rho * cd{cs, sn}, lf, z, f, A.itmax,
-                                           A.work ? &st : nullptr))
+                                           A.work ? &st : nullptr, k > k0))
                 B.fail[s] = 1;
             W.theta[0][k] = th;
             store_pt<DEG>(W.z[0], k, z);
@@ -1426,10 +1432,12 @@ void k_ext_solve_add(ExtArgs A, int cur) {
         sincos(th, &sn, &cs);
         cd z[DEG], left[DEG];
         load_pt<DEG>(W.z[0], a, left);
+#pragma unroll
+        for (int r = 0; r < DEG; ++r) z[r] = left[r];   // start from the left neighbour
         uint32_t f;
         SolveStats st{0, 0, 0, 0};
         if (!ext_solve_point<NL, MODE>(w0 + rho * cd{cs, sn}, lf, z, f, A.itmax,
-                                       A.work ? &st : nullptr))
+                                       A.work ? &st : nullptr, true))
             B.fail[s] = 1;
         if (A.work) ext_count(A.work, st, 1);
         const uint64_t p = greedy_match<DEG>(left, z);
