@@ -459,10 +459,32 @@ __device__ __forceinline__ double ld_brightness_core(cd z, const LensFrame<NL>& 
                                                      double& dBr) {
 #pragma clang fp contract(off)
     const cd zb = conj(z);
+    // v_j = 1/(conj z - conj r_j): the squared moduli share one reciprocal of their product
+    cd d[NL];
+    double n[NL], sj[NL];
+#pragma unroll
+    for (int j = 0; j < NL; ++j) {
+        d[j] = zb - conj(lf.r[j]);
+        n[j] = norm2(d[j]);
+    }
+    if constexpr (NL == 1) {
+        sj[0] = rcp(n[0]);
+    } else if constexpr (NL == 2) {
+        const double rP = rcp(n[0] * n[1]);
+        sj[0] = n[1] * rP;
+        sj[1] = n[0] * rP;
+    } else {
+        const double n01 = n[0] * n[1];
+        const double rP = rcp(n01 * n[2]);
+        const double t = n[2] * rP;
+        sj[0] = n[1] * t;
+        sj[1] = n[0] * t;
+        sj[2] = n01 * rP;
+    }
     cd G = cd{0.0, 0.0};
 #pragma unroll
     for (int j = 0; j < NL; ++j) {
-        v[j] = crcp<2>(zb - conj(lf.r[j]));
+        v[j] = cd{d[j].re * sj[j], -d[j].im * sj[j]};
         G = cd{fma(lf.e[j], v[j].re, G.re), fma(lf.e[j], v[j].im, G.im)};  // explicit: no
     }                                                                       // contraction choice
     D = z - G - w0;
