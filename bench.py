@@ -61,7 +61,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--roots-only", action="store_true")
-    ap.add_argument("--workloads", default="c1,c3,c4,c5,ll",
+    ap.add_argument("--workloads", default="c1,c3,c4,c5,ll,tr",
                     help="comma-separated extra workloads to run after the root solve")
     ap.add_argument("--c5-sets", type=int, default=1024,
                     help="parameter sets of the C5 workload (1024 = the full workload)")
@@ -208,7 +208,8 @@ def main():
                 ("c5", "triple_extended_lightcurves_fwd_vjp",
                  lambda: bench_triple(1, 0, n_sets=args.c5_sets,
                                       cpu_s=0 if args.no_cpu else args.cpu_seconds)),
-                ("ll", "marginalized_log_likelihood_grad", lambda: bench_likelihood(20, 3))):
+                ("ll", "marginalized_log_likelihood_grad", lambda: bench_likelihood(20, 3)),
+                ("tr", "annual_parallax_trajectory_fwd_vjp", lambda: bench_trajectory(20, 3))):
             if key not in sel:
                 continue
             print(f"bench: {name}", file=sys.stderr, flush=True)
@@ -530,6 +531,66 @@ def bench_triple(steps, warmup, n_sets=1024, n_epochs=1500, npts=256, budget_s=4
                  for k in range(n_sets) for e in range(0, n_epochs, 25))
         out["cpu_baseline"] = cpu_extended_baseline(
             items, cpu_s, "C5 epochs (every 25th epoch of the light curves in order)")
+    return out
+
+
+def bench_trajectory(steps, warmup, n_epochs=1500):
+    """Annual-parallax source trajectories (src/caustics/trajectory.py:104-157) for L
+    parameter sets over 1500 epochs: forward (w written, 16 B per curve-epoch) and
+    reverse mode (the cotangent of w read, 16 B per curve-epoch, five sums per curve),
+    L = 1024 and 16384.  The Sun table is an analytic circular orbit on the daily grid
+    (the ephemeris is an input of the kernel).  HIP events on the launch stream."""
+    import torch
+
+    from caustics_amd import _lib
+
+    g = np.random.default_rng(12)
+    t = torch.as_tensor(np.sort(g.uniform(8000.0, 8400.0, n_epochs)), device="cuda")
+    xp_h = np.arange(8000.0, 8401.0, 1.0)
+    om = 2 * np.pi / 365.25
+    tab_h = np.stack([np.sin(om * xp_h), 0.4 * np.cos(om * xp_h), om * np.cos(om * xp_h),
+                      -0.4 * om * np.sin(om * xp_h)])
+    xp = torch.as_tensor(xp_h, device="cuda")
+    tab = torch.as_tensor(tab_h, device="cuda").contiguous()
+    L_ = _lib.lib()
+    out = {}
+    for L in (1024, 16384):
+        P = torch.as_tensor(np.stack([g.uniform(8050, 8350, L), g.uniform(-1, 1, L),
+                                      g.uniform(20, 200, L), g.uniform(-0.5, 0.5, L),
+                                      g.uniform(-0.5, 0.5, L)], 1), device="cuda").contiguous()
+        w = torch.empty((L, n_epochs, 2), dtype=torch.float64, device="cuda")
+        gw = torch.as_tensor(g.normal(size=(L, n_epochs, 2)), device="cuda")
+        gP = torch.empty_like(P)
+        stream = torch.cuda.current_stream()
+        res = {}
+        for kind in ("fwd", "vjp"):
+            def call():
+                rc = L_.caus_trajectory_annual_parallax(
+                    _lib.stream_ptr(), _lib.ptr(t), n_epochs, _lib.ptr(xp), _lib.ptr(tab),
+                    xp_h.size, _lib.ptr(P), L, 0, _lib.ptr(w) if kind == "fwd" else None,
+                    _lib.ptr(gw) if kind == "vjp" else None,
+                    _lib.ptr(gP) if kind == "vjp" else None)
+                assert rc == 0
+
+            for _ in range(warmup):
+                call()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(steps):
+                call()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / steps
+            nbytes = 16.0 * L * n_epochs
+            res[kind] = {"ms": ms, "achieved_GBps": nbytes / ms / 1e6,
+                         "frac": nbytes / ms / 1e6 / 8000.0, "bytes_per_launch": nbytes}
+            print(f"bench: trajectory L={L} {kind}: {ms * 1e3:.1f} us/launch, "
+                  f"{nbytes / ms / 1e6:.0f} GB/s", file=sys.stderr, flush=True)
+        out[f"L{L}"] = {"curves": L, "epochs": n_epochs, **res,
+                        "finite": bool(torch.isfinite(gP).all().item())}
+    out["workload"] = ("annual-parallax trajectories w(t) and d/d(t0, u0, tE, piEE, piEN), "
+                       "1500 epochs, daily Sun table")
     return out
 
 

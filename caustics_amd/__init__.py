@@ -12,6 +12,7 @@ from .linalg import marginalized_log_likelihood
 from .point_source import (critical_and_caustic_curves, images_point_source,
                            images_point_source_sequential, lens_eq, lens_eq_det_jac,
                            mag_point_source)
+from .trajectory import AnnualParallaxTrajectory
 
 __all__ = [
     "ehrlich_aberth",
@@ -29,4 +30,5 @@ __all__ = [
     "mag",
     "mag_fwd_vjp",
     "marginalized_log_likelihood",
+    "AnnualParallaxTrajectory",
 ]

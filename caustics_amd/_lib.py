@@ -58,6 +58,8 @@ SIGNATURES = {
                                                 _vp, _vp, _i32, _i32, _i32]),
     "caus_marginalized_log_likelihood": (ctypes.c_int, [_vp, _vp, _i64, _i64, _i64, _vp, _vp,
                                                         _vp, _i32, _vp, _vp, _vp, _vp, _i64]),
+    "caus_trajectory_annual_parallax": (ctypes.c_int, [_vp, _vp, _i64, _vp, _vp, _i64, _vp,
+                                                       _i64, _i32, _vp, _vp, _vp]),
     "caus_hexadecapole_gate": (ctypes.c_int, [_vp, _i32, _vp, _i64, _vp, _i64, _vp, _i64,
                                               ctypes.c_double, _vp, _vp, _vp, _vp, _i32]),
     "caus_hexadecapole_vjp": (ctypes.c_int, [_vp, _i32, _vp, _i64, _vp, _i64, _vp, _i64,

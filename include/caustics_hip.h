@@ -242,6 +242,25 @@ int caus_marginalized_log_likelihood(void* stream, const double* A, int64_t B, i
                                      const int64_t* seg_off, int32_t nseg, double* ll,
                                      double* beta, const double* g_ll, double* g_A, int64_t ldg);
 
+/* Source trajectories with annual parallax, and their reverse mode.
+ * Replaces AnnualParallaxTrajectory.compute and _compute_delta_sun_position_and_velocity
+ * (src/caustics/trajectory.py:104-157), for L parameter sets at once.
+ *   t (T,) epochs; xp (M,) increasing grid, M >= 2 (the reference's daily t_jpl, :36);
+ *   sun_tab (4, M): rows s_e, s_n, sdot_e, sdot_n, the Sun's position and velocity
+ *     relative to the Earth projected east / north on the plane of the sky (:80-102)
+ *   params (L, 5): t0, u0, tE, then pi_EE, pi_EN (parametrization 0, "cartesian") or
+ *     pi_E, psi (parametrization 1, "polar") (:137-146)
+ *   w (L, T) complex out or NULL: u_e + i u_n (:149-155), the positions interpolated as
+ *     jax.numpy.interp does (clamped to the end values outside xp)
+ *   g_w (L, T) complex cotangents (dL/dRe w + i dL/dIm w) or NULL; with g_w, g_params
+ *     (L, 5) out receives dL/dparams in the order of params.  The cartesian gradient
+ *     is undefined (non-finite) at pi_E = 0, where psi is.
+ * L <= 2^31 - 1. */
+int caus_trajectory_annual_parallax(void* stream, const double* t, int64_t T, const double* xp,
+                                    const double* sun_tab, int64_t M, const double* params,
+                                    int64_t L, int32_t parametrization, double* w,
+                                    const double* g_w, double* g_params);
+
 #ifdef __cplusplus
 }
 #endif

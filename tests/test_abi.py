@@ -44,3 +44,10 @@ def test_argument_errors_enqueue_nothing():
     assert L.caus_roots(None, None, None, None, None, None, 10, 5, 100, 0, 0) == _lib.CAUS_ERR_ARG
     assert L.caus_roots(None, None, None, None, None, None, 0, 5, 100, 0, 0) == _lib.CAUS_OK
     assert L.caus_roots(None, None, None, None, None, None, -1, 5, 100, 0, 0) == _lib.CAUS_ERR_ARG
+    # trajectories: a grid of fewer than two points, an unknown parametrization and a
+    # cotangent without its gradient output are rejected before any HIP call
+    tr = L.caus_trajectory_annual_parallax
+    assert tr(None, None, 10, None, None, 1, None, 4, 0, None, None, None) == _lib.CAUS_ERR_ARG
+    assert tr(None, None, 10, None, None, 5, None, 4, 2, None, None, None) == _lib.CAUS_ERR_ARG
+    assert tr(None, None, 10, None, None, 5, None, 4, 0, None, 8, None) == _lib.CAUS_ERR_ARG
+    assert tr(None, None, 10, None, None, 5, None, 0, 0, None, None, None) == _lib.CAUS_OK
