@@ -14,4 +14,8 @@ Contents
   multipole.py NumPy restatement of the hexadecapole and the gate.
   extended.py  NumPy restatement of the limb sampling / stitching / contour
                integration pipeline.
+  linalg.py    NumPy restatement of the flux-marginalised likelihood.
+  trajectory.py NumPy restatement of the annual-parallax trajectory.
+  mp_lens.py   the point-source route evaluated in 40 significant digits, the
+               truth for clustered roots that FP64 cannot resolve.
 """
