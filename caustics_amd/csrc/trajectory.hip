@@ -10,9 +10,10 @@
 //   u_n    = -u0 sin psi + (t - t0)/tE cos psi + pi_E ds_n
 //   w      = u_e + i u_n                                      (:155)
 // with psi = atan2(pi_EE, pi_EN), pi_E = |(pi_EE, pi_EN)| in the cartesian
-// parametrization (:140-142).  One workgroup per curve: its threads stride over the
-// epochs, write w, and (reverse mode) sum the parameter cotangents in a fixed order
-// (wave shuffles, then the four waves in order), so results are deterministic.  The
+// parametrization (:140-142).  One workgroup per kTrSets parameter sets: its threads
+// stride over the epochs, write w, and (reverse mode) sum the parameter cotangents in a
+// fixed order (wave shuffles, then the four waves in order), so results are
+// deterministic.  The
 // derivative of interp in x is the slope of the segment that gives its value, zero
 // outside the grid, as JAX differentiates it.
 #include <hip/hip_runtime.h>
@@ -22,6 +23,8 @@
 namespace {
 
 constexpr int kTrBlock = 256;
+constexpr int kTrSets = 8;    // most parameter sets per workgroup
+constexpr int kTrEpochs = 8;  // epochs per thread and chunk
 
 struct Seg {
     int64_t i;      // right end of the segment, 1..M-1
@@ -73,83 +76,125 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// Per parameter set: the parameters, the trigonometry of psi and the interpolants at t0
+// with their slopes (the t0 dependence of s(t0) and sdot(t0)).
+struct CurveK {
+    double t0, u0, tE, itE, piE, sp, cp, pEE, pEN;
+    double se0, sn0, sde0, sdn0, kse0, ksn0, ksde0, ksdn0;
+};
+
 // tab: (4, M) rows s_e, s_n, sdot_e, sdot_n; params (L, 5): t0, u0, tE and (pi_EE, pi_EN)
 // or (pi_E, psi); w (L, T) complex; g_w (L, T) complex cotangents (torch convention:
-// dL/dRe + i dL/dIm) or null; g_params (L, 5) out when g_w is given.
+// dL/dRe + i dL/dIm); g_params (L, 5) out (reverse mode).  A workgroup holds `sets`
+// (<= kTrSets) parameter sets and takes the epochs in chunks of kTrBlock * kTrEpochs: each thread
+// locates the grid segments of its kTrEpochs epochs once and evaluates every set of the
+// workgroup there.
+template <bool VJP>
 __global__ __launch_bounds__(kTrBlock) void k_parallax(
     const double* __restrict__ t, int64_t T, const double* __restrict__ xp,
-    const double* __restrict__ tab, int64_t M, const double* __restrict__ params,
-    int polar, double* __restrict__ w, const double* __restrict__ g_w,
+    const double* __restrict__ tab, int64_t M, const double* __restrict__ params, int64_t L,
+    int sets, int polar, double* __restrict__ w, const double* __restrict__ g_w,
     double* __restrict__ g_params) {
-    __shared__ double red[kTrBlock / 64][5];
-    const int64_t c = blockIdx.x;
-    const double* prm = params + c * 5;
-    const double t0 = prm[0], u0 = prm[1], tE = prm[2];
-    double piE, psi;
-    if (polar) {
-        piE = prm[3];
-        psi = prm[4];
-    } else {
-        psi = atan2(prm[3], prm[4]);
-        piE = sqrt(prm[4] * prm[4] + prm[3] * prm[3]);
-    }
-    double sp, cp;
-    sincos(psi, &sp, &cp);
+    __shared__ CurveK ck[kTrSets];
+    __shared__ double red[kTrBlock / 64][kTrSets][5];
+    const int64_t c0 = (int64_t)blockIdx.x * sets;
+    const int nset = (int)(L - c0 < sets ? L - c0 : sets);
     const double *se = tab, *sn = tab + M, *sde = tab + 2 * M, *sdn = tab + 3 * M;
-    const Seg s0 = find_segment(t0, xp, M);
-    double se0, sn0, sde0, sdn0, kse0, ksn0, ksde0, ksdn0;
-    interp_at(s0, se, M, se0, kse0);
-    interp_at(s0, sn, M, sn0, ksn0);
-    interp_at(s0, sde, M, sde0, ksde0);
-    interp_at(s0, sdn, M, sdn0, ksdn0);
-    const double itE = 1.0 / tE;
-    double a_t0 = 0.0, a_u0 = 0.0, a_tE = 0.0, a_piE = 0.0, a_psi = 0.0;
-    for (int64_t e = threadIdx.x; e < T; e += kTrBlock) {
-        const double te = t[e];
-        const Seg st = find_segment(te, xp, M);
-        double set, snt, k1, k2;
-        interp_at(st, se, M, set, k1);
-        interp_at(st, sn, M, snt, k2);
-        const double dt = te - t0;
-        const double dse = set - se0 - dt * sde0;
-        const double dsn = snt - sn0 - dt * sdn0;
-        const double tau = dt / tE;
-        const double ue = u0 * cp + tau * sp + piE * dse;
-        const double un = -u0 * sp + tau * cp + piE * dsn;
-        if (w) {
-            w[2 * (c * T + e)] = ue;
-            w[2 * (c * T + e) + 1] = un;
-        }
-        if (g_w) {
-            const double gr = g_w[2 * (c * T + e)], gi = g_w[2 * (c * T + e) + 1];
-            // d ds / d t0 = -s'(t0) + sdot(t0) - (t - t0) sdot'(t0)
-            const double ddse = -kse0 + sde0 - dt * ksde0;
-            const double ddsn = -ksn0 + sdn0 - dt * ksdn0;
-            a_t0 += gr * (piE * ddse - sp * itE) + gi * (piE * ddsn - cp * itE);
-            a_u0 += gr * cp - gi * sp;
-            a_tE -= (tau * itE) * (gr * sp + gi * cp);
-            a_piE += gr * dse + gi * dsn;
-            a_psi += gr * (tau * cp - u0 * sp) - gi * (u0 * cp + tau * sp);
-        }
-    }
-    if (!g_w) return;
-    double v[5] = {a_t0, a_u0, a_tE, a_piE, a_psi};
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        v[k] = wave_sum(v[k]);
-        if (lane == 0) red[wv][k] = v[k];
+    if (threadIdx.x < nset) {
+        const double* prm = params + (c0 + threadIdx.x) * 5;
+        CurveK k;
+        k.t0 = prm[0];
+        k.u0 = prm[1];
+        k.tE = prm[2];
+        k.itE = 1.0 / k.tE;
+        double psi;
+        k.pEE = prm[3];
+        k.pEN = prm[4];
+        if (polar) {
+            k.piE = prm[3];
+            psi = prm[4];
+        } else {
+            psi = atan2(prm[3], prm[4]);
+            k.piE = sqrt(prm[4] * prm[4] + prm[3] * prm[3]);
+        }
+        sincos(psi, &k.sp, &k.cp);
+        const Seg s0 = find_segment(k.t0, xp, M);
+        interp_at(s0, se, M, k.se0, k.kse0);
+        interp_at(s0, sn, M, k.sn0, k.ksn0);
+        interp_at(s0, sde, M, k.sde0, k.ksde0);
+        interp_at(s0, sdn, M, k.sdn0, k.ksdn0);
+        ck[threadIdx.x] = k;
     }
+    if (VJP && threadIdx.x < (kTrBlock / 64) * kTrSets * 5) (&red[0][0][0])[threadIdx.x] = 0.0;
     __syncthreads();
-    if (threadIdx.x == 0) {
+    for (int64_t e0 = 0; e0 < T; e0 += (int64_t)kTrBlock * kTrEpochs) {
+        double te[kTrEpochs], set[kTrEpochs], snt[kTrEpochs];
+#pragma unroll
+        for (int j = 0; j < kTrEpochs; ++j) {
+            const int64_t e = e0 + threadIdx.x + (int64_t)j * kTrBlock;
+            te[j] = e < T ? t[e] : 0.0;
+            double k1, k2;
+            const Seg st = find_segment(te[j], xp, M);
+            interp_at(st, se, M, set[j], k1);
+            interp_at(st, sn, M, snt[j], k2);
+        }
+        for (int c = 0; c < nset; ++c) {
+            const CurveK k = ck[c];
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;
+#pragma unroll
+            for (int j = 0; j < kTrEpochs; ++j) {
+                const int64_t e = e0 + threadIdx.x + (int64_t)j * kTrBlock;
+                if (e >= T) break;
+                const double dt = te[j] - k.t0;
+                const double dse = set[j] - k.se0 - dt * k.sde0;
+                const double dsn = snt[j] - k.sn0 - dt * k.sdn0;
+                const double tau = dt / k.tE;
+                const int64_t o = 2 * ((c0 + c) * T + e);
+                if (!VJP) {
+                    w[o] = k.u0 * k.cp + tau * k.sp + k.piE * dse;
+                    w[o + 1] = -k.u0 * k.sp + tau * k.cp + k.piE * dsn;
+                } else {
+                    const double gr = g_w[o], gi = g_w[o + 1];
+                    // d ds / d t0 = -s'(t0) + sdot(t0) - (t - t0) sdot'(t0)
+                    const double ddse = -k.kse0 + k.sde0 - dt * k.ksde0;
+                    const double ddsn = -k.ksn0 + k.sdn0 - dt * k.ksdn0;
+                    a0 += gr * (k.piE * ddse - k.sp * k.itE) + gi * (k.piE * ddsn - k.cp * k.itE);
+                    a1 += gr * k.cp - gi * k.sp;
+                    a2 -= (tau * k.itE) * (gr * k.sp + gi * k.cp);
+                    a3 += gr * dse + gi * dsn;
+                    a4 += gr * (tau * k.cp - k.u0 * k.sp) - gi * (k.u0 * k.cp + tau * k.sp);
+                }
+            }
+            if (VJP) {
+                a0 = wave_sum(a0);
+                a1 = wave_sum(a1);
+                a2 = wave_sum(a2);
+                a3 = wave_sum(a3);
+                a4 = wave_sum(a4);
+                if (lane == 0) {
+                    red[wv][c][0] += a0;
+                    red[wv][c][1] += a1;
+                    red[wv][c][2] += a2;
+                    red[wv][c][3] += a3;
+                    red[wv][c][4] += a4;
+                }
+            }
+        }
+    }
+    if (!VJP) return;
+    __syncthreads();
+    if (threadIdx.x < nset) {
+        const int c = threadIdx.x;
         double r[5];
 #pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            r[k] = red[0][k];
+        for (int q = 0; q < 5; ++q) {
+            r[q] = red[0][c][q];
 #pragma unroll
-            for (int q = 1; q < kTrBlock / 64; ++q) r[k] += red[q][k];
+            for (int u = 1; u < kTrBlock / 64; ++u) r[q] += red[u][c][q];
         }
-        double* g = g_params + c * 5;
+        const CurveK& k = ck[c];
+        double* g = g_params + (c0 + c) * 5;
         g[0] = r[0];
         g[1] = r[1];
         g[2] = r[2];
@@ -158,9 +203,9 @@ __global__ __launch_bounds__(kTrBlock) void k_parallax(
             g[4] = r[4];
         } else {
             // psi = atan2(pi_EE, pi_EN), pi_E = |(pi_EE, pi_EN)|; undefined at pi_E = 0
-            const double pEE = prm[3], pEN = prm[4], ip2 = 1.0 / (piE * piE);
-            g[3] = r[3] * (pEE / piE) + r[4] * (pEN * ip2);
-            g[4] = r[3] * (pEN / piE) - r[4] * (pEE * ip2);
+            const double ip2 = 1.0 / (k.piE * k.piE);
+            g[3] = r[3] * (k.pEE / k.piE) + r[4] * (k.pEN * ip2);
+            g[4] = r[3] * (k.pEN / k.piE) - r[4] * (k.pEE * ip2);
         }
     }
 }
@@ -177,7 +222,15 @@ extern "C" int caus_trajectory_annual_parallax(void* stream, const double* t, in
     if (L > 0x7fffffff || (g_w != nullptr) != (g_params != nullptr)) return CAUS_ERR_ARG;
     if (L == 0) return CAUS_OK;
     if (!xp || !sun_tab || !params || (T > 0 && !t) || (!w && !g_w)) return CAUS_ERR_ARG;
-    hipLaunchKernelGGL(k_parallax, dim3((unsigned)L), dim3(kTrBlock), 0, (hipStream_t)stream, t,
-                       T, xp, sun_tab, M, params, parametrization, w, g_w, g_params);
+    // sets per workgroup: as many as keep at least 2048 workgroups (8 per compute unit)
+    const int sets = (int)(L / 2048 < 1 ? 1 : (L / 2048 > kTrSets ? kTrSets : L / 2048));
+    const unsigned blocks = (unsigned)((L + sets - 1) / sets);
+    hipStream_t s = (hipStream_t)stream;
+    if (w)
+        hipLaunchKernelGGL(k_parallax<false>, dim3(blocks), dim3(kTrBlock), 0, s, t, T, xp,
+                           sun_tab, M, params, L, sets, parametrization, w, g_w, g_params);
+    if (g_w)
+        hipLaunchKernelGGL(k_parallax<true>, dim3(blocks), dim3(kTrBlock), 0, s, t, T, xp,
+                           sun_tab, M, params, L, sets, parametrization, w, g_w, g_params);
     return hipGetLastError() == hipSuccess ? CAUS_OK : CAUS_ERR_LAUNCH;
 }
