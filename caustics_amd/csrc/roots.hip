@@ -5,6 +5,8 @@
 // coefficients (N, deg+1) complex128 ascending, optional initial roots (N, deg),
 // output roots (N, deg).  One polynomial per lane, 256 lanes per workgroup, no
 // scratch memory, no allocation, no device synchronisation.
+#include <atomic>
+
 #include "caustics_hip.h"
 #include "roots_impl.hpp"
 
@@ -18,16 +20,19 @@ namespace caus {
 int roots_grid_limit() {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 1024;
-    static int cached[64] = {0};
+    // one value per device; concurrent first calls compute the same value
+    static std::atomic<int> cached[64];
     if (dev < 0 || dev >= 64) return 1024;
-    if (cached[dev] == 0) {
+    int v = cached[dev].load(std::memory_order_relaxed);
+    if (v == 0) {
         int cus = 0;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
             cus <= 0)
             cus = 256;
-        cached[dev] = 8 * cus;
+        v = 8 * cus;
+        cached[dev].store(v, std::memory_order_relaxed);
     }
-    return cached[dev];
+    return v;
 }
 }  // namespace caus
 
