@@ -108,6 +108,120 @@ __global__ __launch_bounds__(kRootsBlock) void k_roots_persistent(
     }
 }
 
+// Fast FP64 form with the converged roots moved behind the unconverged ones at the end
+// of every sweep in which one converged (a stable partition, so the unconverged roots
+// keep their order and a sweep still visits them in the Gauss-Seidel order of
+// EAState::sweep).  The sweep loop then runs root slot j only while some lane has more
+// than j unconverged roots, instead of whenever any lane still needs its own root j.
+// The partition goes through LDS (one column per lane, written at the lane's target
+// slots and read back in slot order); `perm` maps slots to the original root indices
+// for the output.  The Aberth sums add the same terms in slot order, so the roots agree
+// with the uncompacted sweep to rounding.
+template <int DEG>
+struct CompactEA {
+    double al[DEG + 1];  // alpha' weights of the stopping test
+    unsigned cx;         // converged slots; every slot >= na is converged
+    int na;              // unconverged slots at the start of the sweep
+    uint64_t perm;       // slot -> original root index, 4 bits per slot
+
+    __device__ __forceinline__ void start(const cd (&p)[DEG + 1]) {
+#pragma unroll
+        for (int i = 0; i <= DEG; ++i) al[i] = cabs(p[i]) * fma(ALPHA_W, (double)i, 1.0);
+        cx = 0;
+        na = DEG;
+        perm = 0;
+#pragma unroll
+        for (int i = 0; i < DEG; ++i) perm |= (uint64_t)i << (4 * i);
+    }
+    // zs: this lane's LDS column (slot k at zs[k * kRootsBlock])
+    __device__ __forceinline__ bool sweep(const cd (&p)[DEG + 1], cd (&z)[DEG], SolveStats& st,
+                                          double2* zs) {
+        constexpr unsigned ALL = (1u << DEG) - 1u;
+#pragma unroll
+        for (int j = 0; j < DEG; ++j) {
+            if (j < na && !((cx >> j) & 1u)) {
+                ++st.it64;
+                if (fast_step<DEG>(p, al, z, j)) cx |= 1u << j;
+            }
+        }
+        ++st.sweeps;
+        if (cx == ALL) return true;
+        if (cx & ((1u << na) - 1u)) {
+            const unsigned act = ~cx & ALL;
+            const int nact = __builtin_popcount(act);
+            uint64_t np = 0;
+#pragma unroll
+            for (int j = 0; j < DEG; ++j) {
+                const unsigned below = (1u << j) - 1u;
+                const int d = ((act >> j) & 1u) ? __builtin_popcount(act & below)
+                                                : nact + __builtin_popcount(cx & below);
+                zs[d * kRootsBlock] = make_double2(z[j].re, z[j].im);
+                np |= ((perm >> (4 * j)) & 0xFull) << (4 * d);
+            }
+#pragma unroll
+            for (int k = 0; k < DEG; ++k) {
+                const double2 v = zs[k * kRootsBlock];
+                z[k] = cd{v.x, v.y};
+            }
+            perm = np;
+            na = nact;
+            cx = ALL & ~((1u << nact) - 1u);
+        }
+        return false;
+    }
+};
+
+// k_roots_persistent for the Fast FP64 form with CompactEA sweeps.
+template <int DEG>
+__global__ __launch_bounds__(kRootsBlock) void k_roots_compact(
+    const cd* __restrict__ coeffs, const cd* init, cd* roots, int32_t* __restrict__ stats,
+    uint8_t* __restrict__ status, int64_t N, int itmax) {
+    __shared__ double2 zs[DEG * kRootsBlock];
+    const int64_t L = (int64_t)gridDim.x * kRootsBlock;
+    int64_t i = (int64_t)blockIdx.x * kRootsBlock + threadIdx.x;
+    double2* col = zs + threadIdx.x;
+    cd p[DEG + 1], z[DEG];
+    CompactEA<DEG> S;
+    SolveStats st{0, 0, 0, 0};
+    bool active = i < N;
+    if (active) {
+#pragma unroll
+        for (int k = 0; k <= DEG; ++k) p[k] = coeffs[i * (DEG + 1) + k];
+#pragma unroll
+        for (int k = 0; k < DEG; ++k) z[k] = init[i * DEG + k];
+        S.start(p);
+    }
+    while (__ballot(active) != 0ull) {
+        if (active) {
+            const bool done = S.sweep(p, z, st, col);
+            if (done || st.sweeps >= itmax) {
+#pragma unroll
+                for (int k = 0; k < DEG; ++k) roots[i * DEG + (int)((S.perm >> (4 * k)) & 0xF)] = z[k];
+                if (status) status[i] = done ? 1 : 0;
+                if (stats) {
+                    stats[4 * i + 0] = st.sweeps;
+                    stats[4 * i + 1] = st.it64;
+                    stats[4 * i + 2] = 0;
+                    stats[4 * i + 3] = 0;
+                }
+                i += L;
+                active = i < N;
+                if (active) {
+#pragma unroll
+                    for (int k = 0; k <= DEG; ++k) p[k] = coeffs[i * (DEG + 1) + k];
+#pragma unroll
+                    for (int k = 0; k < DEG; ++k) z[k] = init[i * DEG + k];
+                    S.start(p);
+                    st = SolveStats{0, 0, 0, 0};
+                }
+            }
+        }
+    }
+}
+
+// degrees from which the compacted sweep pays for its partition
+constexpr int kRootsCompactMinDeg = 8;
+
 int roots_grid_limit();
 
 template <int DEG, int MODE, bool RC>
@@ -132,6 +246,11 @@ int launch_roots(hipStream_t s, const cd* c, const cd* ri, cd* r, int32_t* st, u
         init = r;
     }
     const int64_t pb = blocks < roots_grid_limit() ? blocks : roots_grid_limit();
+    if (MODE == MODE_FP64 && !RC && DEG >= kRootsCompactMinDeg) {
+        hipLaunchKernelGGL((k_roots_compact<DEG>), dim3((unsigned)pb), dim3(kRootsBlock), 0, s, c,
+                           init, r, st, ok, N, itmax);
+        return hipGetLastError() == hipSuccess ? CAUS_OK : CAUS_ERR_LAUNCH;
+    }
     hipLaunchKernelGGL((k_roots_persistent<DEG, MODE, RC>), dim3((unsigned)pb), dim3(kRootsBlock),
                        0, s, c, init, r, st, ok, N, itmax);
     return hipGetLastError() == hipSuccess ? CAUS_OK : CAUS_ERR_LAUNCH;
