@@ -238,8 +238,10 @@ def main():
                    "init": "complex"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
-                     "kernel": f"k_roots_init<{n}> + k_roots_compact<{n}> (FP64, fast form) "
-                               "(one caus_roots launch, HIP events on its stream)",
+                     "kernel": f"k_roots_init<{n}> + "
+                               + (f"k_roots_compact<{n}>" if n >= 8 else
+                                  f"k_roots_persistent<{n},FP64,fast>")
+                               + " (one caus_roots launch, HIP events on its stream)",
                      "kernel_ms": main_r["kernel_ms"][dom],
                      "flops_per_launch": flops,
                      "flop_rule": f"{flops_per_root_iter(n)} per active FP64 root-iteration"},
