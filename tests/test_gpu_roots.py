@@ -109,6 +109,30 @@ def test_persistent_refill_large_batch(gpu):
     assert ok.all()
 
 
+def test_compacted_sweep_keeps_root_slots(gpu):
+    # degree 10 runs the compacted sweep (converged roots partitioned behind the others
+    # between sweeps); the output must still put every root in the slot of the initial
+    # estimate it came from.  Alternate slots start at the root (they pass the stopping
+    # test at their first step and are partitioned away after sweep 1) or 1e-2 from it
+    # (several sweeps), so every solve runs partitions; each root must come back in its
+    # own slot.
+    g = np.random.default_rng(31)
+    n, deg = 4000, 10
+    zs = g.normal(size=(n, deg)) + 1j * g.normal(size=(n, deg))
+    c = np.array([np.poly(z) for z in zs])[:, ::-1].copy()
+    scale = np.where(np.arange(deg) % 2 == 0, 1e-2, 1e-15)
+    init = zs + scale * (g.normal(size=zs.shape) + 1j * g.normal(size=zs.shape))
+    z, status, stats = solve_with_stats(c, mode="fp64", init="complex", roots_init=init)
+    z, st = z.cpu().numpy(), stats.cpu().numpy()
+    assert status.cpu().numpy().all()
+    assert np.mean(st[:, 0] >= 2) > 0.9          # solves of several sweeps
+    assert np.mean(st[:, 1] < deg * st[:, 0]) > 0.9   # ... with roots converged early
+    err = (np.abs(z - zs) / np.abs(zs)).max(axis=1)
+    assert np.mean(err < 1e-8) > 0.98
+    ok, _, _ = roots_close(z, zs, c, rel=1e-10)
+    assert ok.all()
+
+
 def test_custom_init(gpu):
     c = lens_polys(2, 4000, seed=5)
     z0 = ea.roots_asc(c)
