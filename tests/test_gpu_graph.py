@@ -129,3 +129,35 @@ def test_graph_extended_ld_fwd_vjp(nlenses):
     # the eager call in between uses the same workspace, as a caller reusing it would
     _replay_check(lambda: call(w, mag, gout, st), [mag, gout, st], eager,
                   lambda: call(w2, m2, g2, s2))
+
+
+def test_graph_trajectory_fwd_vjp():
+    g = np.random.default_rng(14)
+    T, M, L = 1500, 401, 64
+    t = torch.as_tensor(np.sort(g.uniform(8000.0, 8400.0, T)), device="cuda")
+    xp_h = np.arange(8000.0, 8401.0, 1.0)
+    om = 2 * np.pi / 365.25
+    tab = torch.as_tensor(np.stack([np.sin(om * xp_h), 0.4 * np.cos(om * xp_h),
+                                    om * np.cos(om * xp_h), -0.4 * om * np.sin(om * xp_h)]),
+                          device="cuda").contiguous()
+    xp = torch.as_tensor(xp_h, device="cuda")
+    P = torch.as_tensor(np.stack([g.uniform(8050, 8350, L), g.uniform(-1, 1, L),
+                                  g.uniform(20, 200, L), g.uniform(-0.5, 0.5, L),
+                                  g.uniform(-0.5, 0.5, L)], 1), device="cuda").contiguous()
+    gw = torch.as_tensor(g.normal(size=(L, T, 2)), device="cuda")
+    w = torch.empty((L, T, 2), dtype=torch.float64, device="cuda")
+    gP = torch.empty_like(P)
+    Lb = _lib.lib()
+
+    def call(PP, ww, gg):
+        assert Lb.caus_trajectory_annual_parallax(_lib.stream_ptr(), _lib.ptr(t), T, _lib.ptr(xp),
+                                                  _lib.ptr(tab), M, _lib.ptr(PP), L, 0,
+                                                  _lib.ptr(ww), _lib.ptr(gw), _lib.ptr(gg)) == 0
+
+    call(P, w, gP)
+    torch.cuda.synchronize()
+    eager = [w.clone(), gP.clone()]
+    P2 = P.clone()
+    P2[:, 1] += 0.1
+    w2, g2 = torch.empty_like(w), torch.empty_like(gP)
+    _replay_check(lambda: call(P, w, gP), [w, gP], eager, lambda: call(P2, w2, g2))
