@@ -171,7 +171,8 @@ struct CompactEA {
     }
 };
 
-// k_roots_persistent for the Fast FP64 form with CompactEA sweeps.
+// k_roots_persistent for the Fast FP64 form with CompactEA sweeps.  Launched with one
+// polynomial per lane; lanes refill (stride L) only if the grid had to be capped.
 template <int DEG>
 __global__ __launch_bounds__(kRootsBlock) void k_roots_compact(
     const cd* __restrict__ coeffs, const cd* init, cd* roots, int32_t* __restrict__ stats,
@@ -220,7 +221,7 @@ __global__ __launch_bounds__(kRootsBlock) void k_roots_compact(
 }
 
 // degrees from which the compacted sweep pays for its partition
-constexpr int kRootsCompactMinDeg = 8;
+constexpr int kRootsCompactMinDeg = 5;
 
 int roots_grid_limit();
 
@@ -247,7 +248,11 @@ int launch_roots(hipStream_t s, const cd* c, const cd* ri, cd* r, int32_t* st, u
     }
     const int64_t pb = blocks < roots_grid_limit() ? blocks : roots_grid_limit();
     if (MODE == MODE_FP64 && !RC && DEG >= kRootsCompactMinDeg) {
-        hipLaunchKernelGGL((k_roots_compact<DEG>), dim3((unsigned)pb), dim3(kRootsBlock), 0, s, c,
+        // one polynomial per lane: a partition helps while the lanes of a wave are at
+        // similar stages, which refilled lanes are not; the workgroup dispatcher then
+        // balances the load (measured: 4% faster at degree 5 than with refill)
+        const int64_t cb = blocks < 0x7fffffff ? blocks : 0x7fffffff;
+        hipLaunchKernelGGL((k_roots_compact<DEG>), dim3((unsigned)cb), dim3(kRootsBlock), 0, s, c,
                            init, r, st, ok, N, itmax);
         return hipGetLastError() == hipSuccess ? CAUS_OK : CAUS_ERR_LAUNCH;
     }
