@@ -124,9 +124,11 @@ struct CompactEA {
     int na;              // unconverged slots at the start of the sweep
     uint64_t perm;       // slot -> original root index, 4 bits per slot
 
-    __device__ __forceinline__ void start(const cd (&p)[DEG + 1]) {
+    // a[i] = |p_i|; z gets the initial estimates of init_est unless given
+    __device__ __forceinline__ void start(const double (&a)[DEG + 1], cd (&z)[DEG], bool given) {
+        if (!given) init_est<DEG, false>(a, z);
 #pragma unroll
-        for (int i = 0; i <= DEG; ++i) al[i] = cabs(p[i]) * fma(ALPHA_W, (double)i, 1.0);
+        for (int i = 0; i <= DEG; ++i) al[i] = a[i] * fma(ALPHA_W, (double)i, 1.0);
         cx = 0;
         na = DEG;
         perm = 0;
@@ -172,7 +174,14 @@ struct CompactEA {
 };
 
 // k_roots_persistent for the Fast FP64 form with CompactEA sweeps.  Launched with one
-// polynomial per lane; lanes refill (stride L) only if the grid had to be capped.
+// polynomial per lane; lanes refill (stride L) only if the grid had to be capped.  With
+// init null every lane computes its own initial estimates (init_est.h:57-102, the same
+// function and inputs as k_roots_init, so the same doubles): all lanes start together,
+// so this adds no divergence, and no estimates pass through HBM.  Inlined, the estimate
+// code raises the kernel's register allocation (degree 10: 249 VGPRs, 2 waves per
+// SIMD, against 166 and 3 without it), yet the fused launch measured 4% (degree 10) and
+// 7% (degree 5) faster than the init kernel plus this one; a called, non-inlined
+// init_est was more than twice as slow (its stack frame goes through scratch).
 template <int DEG>
 __global__ __launch_bounds__(kRootsBlock) void k_roots_compact(
     const cd* __restrict__ coeffs, const cd* init, cd* roots, int32_t* __restrict__ stats,
@@ -185,13 +194,20 @@ __global__ __launch_bounds__(kRootsBlock) void k_roots_compact(
     CompactEA<DEG> S;
     SolveStats st{0, 0, 0, 0};
     bool active = i < N;
-    if (active) {
+    auto load = [&](int64_t q) {
+        double a[DEG + 1];
 #pragma unroll
-        for (int k = 0; k <= DEG; ++k) p[k] = coeffs[i * (DEG + 1) + k];
+        for (int k = 0; k <= DEG; ++k) {
+            p[k] = coeffs[q * (DEG + 1) + k];
+            a[k] = cabs(p[k]);
+        }
+        if (init) {
 #pragma unroll
-        for (int k = 0; k < DEG; ++k) z[k] = init[i * DEG + k];
-        S.start(p);
-    }
+            for (int k = 0; k < DEG; ++k) z[k] = init[q * DEG + k];
+        }
+        S.start(a, z, init != nullptr);
+    };
+    if (active) load(i);
     while (__ballot(active) != 0ull) {
         if (active) {
             const bool done = S.sweep(p, z, st, col);
@@ -208,11 +224,7 @@ __global__ __launch_bounds__(kRootsBlock) void k_roots_compact(
                 i += L;
                 active = i < N;
                 if (active) {
-#pragma unroll
-                    for (int k = 0; k <= DEG; ++k) p[k] = coeffs[i * (DEG + 1) + k];
-#pragma unroll
-                    for (int k = 0; k < DEG; ++k) z[k] = init[i * DEG + k];
-                    S.start(p);
+                    load(i);
                     st = SolveStats{0, 0, 0, 0};
                 }
             }
@@ -240,6 +252,16 @@ int launch_roots(hipStream_t s, const cd* c, const cd* ri, cd* r, int32_t* st, u
                            s, c, ri, r, st, ok, N, itmax);
         return hipGetLastError() == hipSuccess ? CAUS_OK : CAUS_ERR_LAUNCH;
     }
+    if (MODE == MODE_FP64 && !RC && DEG >= kRootsCompactMinDeg) {
+        // one polynomial per lane: a partition helps while the lanes of a wave are at
+        // similar stages, which refilled lanes are not; the workgroup dispatcher then
+        // balances the load (measured: 4% faster at degree 5 than with refill).  The
+        // kernel computes the initial estimates itself when none are given.
+        const int64_t cb = blocks < 0x7fffffff ? blocks : 0x7fffffff;
+        hipLaunchKernelGGL((k_roots_compact<DEG>), dim3((unsigned)cb), dim3(kRootsBlock), 0, s, c,
+                           ri, r, st, ok, N, itmax);
+        return hipGetLastError() == hipSuccess ? CAUS_OK : CAUS_ERR_LAUNCH;
+    }
     const cd* init = ri;
     if (!ri) {
         hipLaunchKernelGGL((k_roots_init<DEG, RC>), dim3((unsigned)blocks), dim3(kRootsBlock), 0, s,
@@ -247,15 +269,6 @@ int launch_roots(hipStream_t s, const cd* c, const cd* ri, cd* r, int32_t* st, u
         init = r;
     }
     const int64_t pb = blocks < roots_grid_limit() ? blocks : roots_grid_limit();
-    if (MODE == MODE_FP64 && !RC && DEG >= kRootsCompactMinDeg) {
-        // one polynomial per lane: a partition helps while the lanes of a wave are at
-        // similar stages, which refilled lanes are not; the workgroup dispatcher then
-        // balances the load (measured: 4% faster at degree 5 than with refill)
-        const int64_t cb = blocks < 0x7fffffff ? blocks : 0x7fffffff;
-        hipLaunchKernelGGL((k_roots_compact<DEG>), dim3((unsigned)cb), dim3(kRootsBlock), 0, s, c,
-                           init, r, st, ok, N, itmax);
-        return hipGetLastError() == hipSuccess ? CAUS_OK : CAUS_ERR_LAUNCH;
-    }
     hipLaunchKernelGGL((k_roots_persistent<DEG, MODE, RC>), dim3((unsigned)pb), dim3(kRootsBlock),
                        0, s, c, init, r, st, ok, N, itmax);
     return hipGetLastError() == hipSuccess ? CAUS_OK : CAUS_ERR_LAUNCH;
