@@ -238,9 +238,8 @@ def main():
                    "init": "complex"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
-                     "kernel": f"k_roots_init<{n}> + "
-                               + (f"k_roots_compact<{n}>" if n >= 5 else
-                                  f"k_roots_persistent<{n},FP64,fast>")
+                     "kernel": (f"k_roots_compact<{n}> (initial estimates in-kernel)" if n >= 5 else
+                                f"k_roots_init<{n}> + k_roots_persistent<{n},FP64,fast>")
                                + " (one caus_roots launch, HIP events on its stream)",
                      "kernel_ms": main_r["kernel_ms"][dom],
                      "flops_per_launch": flops,
