@@ -13,10 +13,12 @@
 using namespace caus;
 
 namespace caus {
-// Workgroups of the persistent root kernels: 8 per compute unit of the current device.
-// That is more than fit at once (4-5 at the degree-5 kernel's occupancy, 3 at degree 10);
-// the surplus starts as resident workgroups finish and takes its static share of the
-// batch, which measured a few percent faster than a grid of exactly the resident count.
+// Workgroups of the lane-refill root kernels (k_roots_persistent: reference-compatible
+// and compensated forms, and the Fast form below degree 5): 8 per compute unit of the
+// current device.  That is more than fit at once (3-5 per compute unit, by degree and
+// form); the surplus starts as resident workgroups finish and takes its static share of
+// the batch, which measured a few percent faster than a grid of exactly the resident
+// count.  The compacted Fast-form kernel runs one polynomial per lane instead.
 int roots_grid_limit() {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 1024;
