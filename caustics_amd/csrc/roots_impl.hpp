@@ -178,7 +178,7 @@ struct CompactEA {
 // init null every lane computes its own initial estimates (init_est.h:57-102, the same
 // function and inputs as k_roots_init, so the same doubles): all lanes start together,
 // so this adds no divergence, and no estimates pass through HBM.  Inlined, the estimate
-// code raises the kernel's register allocation (degree 10: 249 VGPRs, 2 waves per
+// code raises the kernel's register allocation (degree 10: 186 VGPRs, 2 waves per
 // SIMD, against 166 and 3 without it), yet the fused launch measured 4% (degree 10) and
 // 7% (degree 5) faster than the init kernel plus this one; a called, non-inlined
 // init_est was more than twice as slow (its stack frame goes through scratch).

@@ -77,12 +77,14 @@ __device__ __forceinline__ void init_est(const double (&alpha)[DEG + 1], cd (&z)
         int j = m - xlo;
         int e = __builtin_popcount(st >> (xhi + 1));
         double r = exp((sel(y, xlo) - sel(y, xhi)) / nz);
-        double t = (pi2 / nz) * j + th * e + sigma;
         if (REALAXIS) {
+            const double t = (pi2 / nz) * j + th * e + sigma;
             z[m] = cd{r * sin(t), 0.0};
         } else {
+            // the same angle in units of pi: sincospi needs no long argument reduction,
+            // which keeps this code small where it is inlined into the sweep kernel
             double s, c;
-            sincos(t, &s, &c);
+            sincospi((2.0 / nz) * j + (2.0 / DEG) * e + sigma * 0.31830988618379067154, &s, &c);
             z[m] = cd{r * c, r * s};
         }
     }
