@@ -17,7 +17,7 @@
 // the explicit dependence on A_i counts) plus the derivative of -log det N / 2:
 //   d ll / d A_i = c_i (F_s r_i - a_i / Saa).
 // One wavefront per curve walks its segments in order; every reduction is a fixed-order
-// wave sum, so results are deterministic.  The magnifications of the first 512 epochs of
+// wave sum, so results are deterministic.  The magnifications of the first 1024 epochs of
 // a segment are read from HBM once and held in registers for the three passes (later
 // epochs are read again, mostly from cache); g_A is written once; f and c are shared by
 // every curve and stay in cache.
@@ -29,7 +29,7 @@ namespace {
 
 constexpr int kLLBlock = 256;               // 4 waves, one curve per wave
 constexpr int kLLCurves = kLLBlock / 64;
-constexpr int kLLCache = 8;                 // epochs of A per lane held in registers
+constexpr int kLLCache = 16;                // epochs of A per lane held in registers
 constexpr double kTwoPi = 6.28318530717958647692;
 
 __device__ __forceinline__ double wave_sum64(double v) {
