@@ -7,7 +7,9 @@
 //   compensated stage ........ lib/ehrlich_aberth/ehrlich_aberth.h:153-292,
 //                              lib/ehrlich_aberth/horner.h:281-385
 // Layout: coefficients ascending (p[k] multiplies z^k), held in VGPRs; every
-// loop over the degree is unrolled at compile time so no array leaves registers.
+// loop over the degree is unrolled at compile time, so the iteration's arrays stay in
+// registers (init_est's log-magnitude table, read at runtime hull vertices, is the
+// one array the compiler places in scratch).
 //
 // Two iteration forms:
 //   RefCompat  the reference's branch structure: |z|>1 uses the reversed polynomial
