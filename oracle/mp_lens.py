@@ -129,3 +129,20 @@ def roots_mp(coeffs_asc):
                              extraprec=4 * DPS)
             out[i] = [complex(x) for x in z]
     return out
+
+
+def mag_point_source_mpf(w, nlenses, **params):
+    """Point-source magnification at one source position w (user frame; an mpmath
+    number or a complex taken as exact), returned as an mpmath number at the working
+    precision (no rounding to FP64), so that finite-difference stencils of it can use
+    small steps.  Call inside mp.workdps(DPS)."""
+    r, e, x_cm = frame_mp(nlenses, **params)
+    wl = mp.mpc(w) + x_cm
+    z = mp.polyroots(poly_mp(wl, r, e), maxsteps=400, extraprec=4 * mp.mp.dps)
+    m = mp.mpf(0)
+    for zi in z:
+        zb = mp.conj(zi)
+        if abs(zi - sum(ej / (zb - mp.conj(rj)) for ej, rj in zip(e, r)) - wl) < 1e-6:
+            K = sum(ej / (zb - mp.conj(rj)) ** 2 for ej, rj in zip(e, r))
+            m += 1 / abs(1 - abs(K) ** 2)
+    return m

@@ -114,20 +114,28 @@ def mag_hexadecapole(z, mask, rho, u1=0.0, nlenses=2, **lp):
     return mu_multi, np.sum(mask * np.abs(dq), axis=0) + np.sum(mask * np.abs(dh), axis=0)
 
 
+def lens_map_f(x, a, e1):
+    """f of the binary lens map w = z + f(conj z) (lightcurve.py:30)."""
+    return -e1 / (x - a) - (1 - e1) / (x + a)
+
+
+def lens_map_fp(x, a, e1):
+    """f' (lightcurve.py:31)."""
+    return e1 / (x - a) ** 2 + (1 - e1) / (x + a) ** 2
+
+
+def lens_map_fpp(x, a, e1):
+    """f'' (lightcurve.py:32)."""
+    return 2 * (e1 / (a - x) ** 3 - (1 - e1) / (a + x) ** 3)
+
+
 def caustics_proximity_test(w, z, mask, rho, dmu, nlenses=2, c_m=1e-2, gamma=0.02, c_f=4.0,
                             rho_min=1e-3, **lp):
     """lightcurve.py:23-75 (binary branch)."""
     a, e1 = lp["a"], lp["e1"]
-
-    def f(x):
-        return -e1 / (x - a) - (1 - e1) / (x + a)
-
-    def fp(x):
-        return e1 / (x - a) ** 2 + (1 - e1) / (x + a) ** 2
-
-    def fpp(x):
-        return 2 * (e1 / (a - x) ** 3 - (1 - e1) / (a + x) ** 3)
-
+    f = lambda x: lens_map_f(x, a, e1)
+    fp = lambda x: lens_map_fp(x, a, e1)
+    fpp = lambda x: lens_map_fpp(x, a, e1)
     zb = cj(z)
     zhat = cj(w) - f(z)
     fp_z, fpp_z, fp_zb, fp_zh, fpp_zb = fp(z), fpp(z), fp(zb), fp(zhat), fpp(zb)
