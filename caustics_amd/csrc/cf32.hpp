@@ -42,7 +42,8 @@ __device__ __forceinline__ cf aberth_sum_f(const cf (&z)[DEG], int j, cf x) {
     return S;
 }
 
-template <int DEG>
+// GUARD: the public root solver's guarded FP64 stage (fast_step)
+template <int DEG, bool GUARD = false>
 __device__ __forceinline__ bool solve_cf32(const cd (&p)[DEG + 1], cd (&z)[DEG], int itmax,
                                            bool custom_init, SolveStats& st) {
     st.sweeps = 0;
@@ -62,7 +63,7 @@ __device__ __forceinline__ bool solve_cf32(const cd (&p)[DEG + 1], cd (&z)[DEG],
         double re = p[k].re * sc, im = p[k].im * sc;
         ph[k] = cf{(float)re, (float)im};
         pl[k] = cf{(float)(re - (double)ph[k].re), (float)(im - (double)ph[k].im)};
-        al64[k] = cabs(p[k]);
+        al64[k] = GUARD ? cabs_safe(p[k]) : cabs(p[k]);
         al[k] = (float)(al64[k] * sc) * (float)fma(ALPHA_W, (double)k, 1.0);
     }
     if (!custom_init) init_est<DEG, false>(al64, z);
@@ -146,7 +147,7 @@ __device__ __forceinline__ bool solve_cf32(const cd (&p)[DEG + 1], cd (&z)[DEG],
         for (int j = 0; j < DEG; ++j) {
             if (!((cx >> j) & 1u)) {
                 ++st.it64;
-                if (fast_step<DEG>(p, al64, z, j)) cx |= 1u << j;
+                if (fast_step<DEG, GUARD>(p, al64, z, j)) cx |= 1u << j;
             }
         }
         if (cx == ALL) {
@@ -164,7 +165,7 @@ __device__ __forceinline__ bool solve_cf32(const cd (&p)[DEG + 1], cd (&z)[DEG],
     // pass the stopping test): the polynomial is solved again by the FP64 form.
     if (!finite) {
         SolveStats s2;
-        done = solve_fp64<DEG, MODE_FP64, false>(p, z, itmax, false, s2);
+        done = solve_fp64<DEG, MODE_FP64, false, GUARD>(p, z, itmax, false, s2);
         st.sweeps = it + s2.sweeps;
         st.it64 += s2.it64;
     }

@@ -57,6 +57,33 @@ __device__ __forceinline__ cd cdiv(cd a, cd b) {
 }
 __device__ __forceinline__ double cabs(cd z) { return sqrt(norm2(z)); }
 
+// 2^-e with e the exponent of m > 0 (m * pow2_inv(m) lies in [1, 2)): a scale that is
+// exact to apply, for squares that would underflow or overflow
+__device__ __forceinline__ double pow2_inv(double m) { return ldexp(1.0, -ilogb(m)); }
+__device__ __forceinline__ double cmaxabs(cd z) { return fmax(fabs(z.re), fabs(z.im)); }
+// 1/z and a/b with both operands scaled by a power of two first (no underflow or
+// overflow of |b|^2 for any finite non-zero b)
+// |z| without underflow or overflow of the squares (accurate to about an ulp)
+__device__ __forceinline__ double cabs_safe(cd z) {
+    const double m = cmaxabs(z);
+    if (!(m > 0.0) || !isfinite(m)) return m;
+    const int e = ilogb(m);
+    const double s = ldexp(1.0, -e);
+    return ldexp(sqrt(norm2(cd{z.re * s, z.im * s})), e);
+}
+__device__ __forceinline__ cd crcp_scaled(cd z) {
+    const double s = pow2_inv(cmaxabs(z));
+    const cd t = cd{z.re * s, z.im * s};
+    const double r = s / norm2(t);
+    return cd{t.re * r, -t.im * r};
+}
+__device__ __forceinline__ cd cdiv_scaled(cd a, cd b) {
+    const double s = pow2_inv(cmaxabs(b));
+    const cd t = cd{b.re * s, b.im * s}, u = cd{a.re * s, a.im * s};
+    const double r = 1.0 / norm2(t);
+    return cd{fma(u.re, t.re, u.im * t.im) * r, fma(u.im, t.re, -u.re * t.im) * r};
+}
+
 // ---- FP32 -------------------------------------------------------------------
 __device__ __forceinline__ cf operator+(cf a, cf b) { return cf{a.re + b.re, a.im + b.im}; }
 __device__ __forceinline__ cf operator-(cf a, cf b) { return cf{a.re - b.re, a.im - b.im}; }

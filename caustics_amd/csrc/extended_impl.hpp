@@ -166,16 +166,35 @@ __device__ __forceinline__ uint64_t nn_map(const cd (&a)[DEG], const cd (&b)[DEG
     return conflict ? (p | kConflictBit) : p;
 }
 
+// roots in lexicographic order of (re, im): an odd-even transposition network with
+// compile-time indices (the array stays in registers)
+template <int DEG>
+__device__ __forceinline__ void sort_roots(cd (&z)[DEG]) {
+#pragma unroll
+    for (int pass = 0; pass < DEG; ++pass) {
+#pragma unroll
+        for (int i = pass & 1; i + 1 < DEG; i += 2) {
+            const bool sw = z[i + 1].re < z[i].re || (z[i + 1].re == z[i].re && z[i + 1].im < z[i].im);
+            const cd lo = sw ? z[i + 1] : z[i], hi = sw ? z[i] : z[i + 1];
+            z[i] = lo;
+            z[i + 1] = hi;
+        }
+    }
+}
+
 // ---- point solve -----------------------------------------------------------
 // warm = true: z holds initial roots (those of a nearby limb point).  The solve ends
 // with the compensated polish, which takes converged roots to the doubles nearest the
-// exact roots, so the result does not depend on where the iteration started (except for
-// clustered roots) and the rows are put in order afterwards by value (match_points).
+// exact roots, so the result does not depend on where the iteration started and the
+// rows are put in order afterwards by value (match_points).  canon = true sorts the
+// roots by value: the first limb point's row order seeds the continuation of every row
+// along the limb (greedy matching resolves competing images in row order), so it must
+// not depend on the iteration either.
 template <int NL, int MODE>
 __device__ __forceinline__ bool ext_solve_point(cd w, const LensFrame<NL>& lf,
                                                 cd (&z)[ext_degree<NL>()], uint32_t& flags,
                                                 int itmax, SolveStats* stats = nullptr,
-                                                bool warm = false) {
+                                                bool warm = false, bool canon = false) {
     constexpr int DEG = ext_degree<NL>();
     bool ok = true;
     flags = 0;
@@ -191,6 +210,7 @@ __device__ __forceinline__ bool ext_solve_point(cd w, const LensFrame<NL>& lf,
         }
     } else {
         ok = solve_lens<NL, MODE, true>(w, lf, z, itmax, warm, stats);
+        if (canon) sort_roots<DEG>(z);
 #pragma unroll
         for (int j = 0; j < DEG; ++j) {
             ImageTerms<NL> it = image_terms<NL>(z[j], lf);
@@ -1247,11 +1267,12 @@ static __global__ __launch_bounds__(kExtPlanBlock) void k_ext_plan(ExtArgs A) {
 }
 
 // ---- initial limb points (extended_source.py:100-106): one run of kExtPer consecutive
-// angles per lane.  Every point starts from the standard initial estimates: a start
-// from a neighbour's images can leave two estimates inside one cluster of nearly double
-// images (near a caustic), both passing the backward-error stopping test, so that one
-// image is found twice and its partner never; independent solves keep the image sets
-// of the oracle.
+// angles per lane.  The lane's first angle starts from the standard initial estimates,
+// each later one from the previous angle's roots.  A warm start can leave two estimates
+// inside one cluster of nearly double images (near a caustic), where the FP64 stopping
+// test accepts both before they separate; the compensated polish that ends every solve
+// continues the Aberth iteration there and takes them to the cluster's distinct roots.
+// Limb point 0 is stored with its roots sorted by value (ext_solve_point, canon).
 template <int NL, int MODE>
 __global__ __launch_bounds__(kExtFlatBlock)
 __attribute__((amdgpu_waves_per_eu(2)))  // 2 waves/SIMD measured faster than 1
@@ -1284,7 +1305,7 @@ void k_ext_limb_init(ExtArgs A) {
             uint32_t f;
             // the lane's later angles start from the previous angle's roots
             if (!ext_solve_point<NL, MODE>(w0 + rho * cd{cs, sn}, lf, z, f, A.itmax,
-                                           A.work ? &st : nullptr, k > k0))
+                                           A.work ? &st : nullptr, k > k0, k == 0))
                 B.fail[s] = 1;
             W.theta[0][k] = th;
             store_pt<DEG>(W.z[0], k, z);

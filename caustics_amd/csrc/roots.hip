@@ -4,7 +4,8 @@
 // cpu_ops.cc:15-81) and gpu_ehrlich_aberth (lib/ehrlich_aberth/kernels.cc.cu:52-104):
 // coefficients (N, deg+1) complex128 ascending, optional initial roots (N, deg),
 // output roots (N, deg).  One polynomial per lane, 256 lanes per workgroup, no
-// scratch memory, no allocation, no device synchronisation.
+// allocation, no device synchronisation.  The only scratch memory is init_est's
+// log-magnitude table (read at runtime hull vertices), (deg+1) doubles per lane.
 #include <atomic>
 
 #include "caustics_hip.h"

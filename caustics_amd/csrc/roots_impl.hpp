@@ -16,11 +16,12 @@ __global__ __launch_bounds__(kRootsBlock) void k_roots(const cd* __restrict__ co
                                                        int32_t* __restrict__ stats,
                                                        uint8_t* __restrict__ status, int64_t N,
                                                        int itmax) {
-    const int64_t i = (int64_t)blockIdx.x * kRootsBlock + threadIdx.x;
-    if (i >= N) return;
+    for (int64_t i = (int64_t)blockIdx.x * kRootsBlock + threadIdx.x; i < N;
+         i += (int64_t)gridDim.x * kRootsBlock) {
     cd p[DEG + 1];
 #pragma unroll
     for (int k = 0; k <= DEG; ++k) p[k] = coeffs[i * (DEG + 1) + k];
+    prescale_coeffs<DEG>(p);
     cd z[DEG];
     const bool custom = roots_init != nullptr;
     if (custom) {
@@ -30,9 +31,9 @@ __global__ __launch_bounds__(kRootsBlock) void k_roots(const cd* __restrict__ co
     SolveStats st;
     bool ok;
     if (MODE == MODE_CF32)
-        ok = solve_cf32<DEG>(p, z, itmax, custom, st);
+        ok = solve_cf32<DEG, true>(p, z, itmax, custom, st);
     else
-        ok = solve_fp64<DEG, MODE, REFCOMPAT>(p, z, itmax, custom, st);
+        ok = solve_fp64<DEG, MODE, REFCOMPAT, true>(p, z, itmax, custom, st);
 #pragma unroll
     for (int k = 0; k < DEG; ++k) roots[i * DEG + k] = z[k];
     if (status) status[i] = ok ? 1 : 0;
@@ -42,21 +43,27 @@ __global__ __launch_bounds__(kRootsBlock) void k_roots(const cd* __restrict__ co
         stats[4 * i + 2] = st.itcomp;
         stats[4 * i + 3] = st.it32;
     }
+    }
 }
 
 // Initial estimates for every polynomial (init_est.h:57-102), written to `roots`.
 template <int DEG, bool REFCOMPAT>
 __global__ __launch_bounds__(kRootsBlock) void k_roots_init(const cd* __restrict__ coeffs,
                                                             cd* __restrict__ roots, int64_t N) {
-    const int64_t i = (int64_t)blockIdx.x * kRootsBlock + threadIdx.x;
-    if (i >= N) return;
+    for (int64_t i = (int64_t)blockIdx.x * kRootsBlock + threadIdx.x; i < N;
+         i += (int64_t)gridDim.x * kRootsBlock) {
+    cd p[DEG + 1];
+#pragma unroll
+    for (int k = 0; k <= DEG; ++k) p[k] = coeffs[i * (DEG + 1) + k];
+    prescale_coeffs<DEG>(p);
     double al[DEG + 1];
 #pragma unroll
-    for (int k = 0; k <= DEG; ++k) al[k] = cabs(coeffs[i * (DEG + 1) + k]);
+    for (int k = 0; k <= DEG; ++k) al[k] = cabs_safe(p[k]);
     cd z[DEG];
     init_est<DEG, REFCOMPAT>(al, z);
 #pragma unroll
     for (int k = 0; k < DEG; ++k) roots[i * DEG + k] = z[k];
+    }
 }
 
 // Persistent solve with lane refill: each lane walks the polynomials i, i + L, i + 2L,
@@ -70,12 +77,13 @@ __global__ __launch_bounds__(kRootsBlock) void k_roots_persistent(
     const int64_t L = (int64_t)gridDim.x * kRootsBlock;
     int64_t i = (int64_t)blockIdx.x * kRootsBlock + threadIdx.x;
     cd p[DEG + 1], z[DEG];
-    EAState<DEG, MODE, REFCOMPAT> S;
+    EAState<DEG, MODE, REFCOMPAT, true> S;
     SolveStats st{0, 0, 0, 0};
     bool active = i < N;
     if (active) {
 #pragma unroll
         for (int k = 0; k <= DEG; ++k) p[k] = coeffs[i * (DEG + 1) + k];
+        prescale_coeffs<DEG>(p);
 #pragma unroll
         for (int k = 0; k < DEG; ++k) z[k] = init[i * DEG + k];
         S.start(p, z, true);
@@ -98,6 +106,7 @@ __global__ __launch_bounds__(kRootsBlock) void k_roots_persistent(
                 if (active) {
 #pragma unroll
                     for (int k = 0; k <= DEG; ++k) p[k] = coeffs[i * (DEG + 1) + k];
+                    prescale_coeffs<DEG>(p);
 #pragma unroll
                     for (int k = 0; k < DEG; ++k) z[k] = init[i * DEG + k];
                     S.start(p, z, true);
@@ -143,7 +152,7 @@ struct CompactEA {
         for (int j = 0; j < DEG; ++j) {
             if (j < na && !((cx >> j) & 1u)) {
                 ++st.it64;
-                if (fast_step<DEG>(p, al, z, j)) cx |= 1u << j;
+                if (fast_step<DEG, true>(p, al, z, j)) cx |= 1u << j;
             }
         }
         ++st.sweeps;
@@ -197,10 +206,10 @@ __global__ __launch_bounds__(kRootsBlock) void k_roots_compact(
     auto load = [&](int64_t q) {
         double a[DEG + 1];
 #pragma unroll
-        for (int k = 0; k <= DEG; ++k) {
-            p[k] = coeffs[q * (DEG + 1) + k];
-            a[k] = cabs(p[k]);
-        }
+        for (int k = 0; k <= DEG; ++k) p[k] = coeffs[q * (DEG + 1) + k];
+        prescale_coeffs<DEG>(p);
+#pragma unroll
+        for (int k = 0; k <= DEG; ++k) a[k] = cabs_safe(p[k]);
         if (init) {
 #pragma unroll
             for (int k = 0; k < DEG; ++k) z[k] = init[q * DEG + k];
@@ -237,10 +246,15 @@ constexpr int kRootsCompactMinDeg = 5;
 
 int roots_grid_limit();
 
+// HIP limits a launch to 2^32 - 1 work-items; every root kernel strides over the batch
+// (grid-stride or lane refill), so its grid is capped there.
+constexpr int64_t kRootsMaxBlocks = 0xFFFFFFFFll / kRootsBlock;
+
 template <int DEG, int MODE, bool RC>
 int launch_roots(hipStream_t s, const cd* c, const cd* ri, cd* r, int32_t* st, uint8_t* ok,
                  int64_t N, int itmax) {
-    const int64_t blocks = (N + kRootsBlock - 1) / kRootsBlock;
+    const int64_t nb = (N + kRootsBlock - 1) / kRootsBlock;
+    const int64_t blocks = nb < kRootsMaxBlocks ? nb : kRootsMaxBlocks;
     if (MODE == MODE_CF32) {
         hipLaunchKernelGGL((k_roots<DEG, MODE, RC>), dim3((unsigned)blocks), dim3(kRootsBlock), 0,
                            s, c, ri, r, st, ok, N, itmax);
@@ -257,8 +271,7 @@ int launch_roots(hipStream_t s, const cd* c, const cd* ri, cd* r, int32_t* st, u
         // similar stages, which refilled lanes are not; the workgroup dispatcher then
         // balances the load (measured: 4% faster at degree 5 than with refill).  The
         // kernel computes the initial estimates itself when none are given.
-        const int64_t cb = blocks < 0x7fffffff ? blocks : 0x7fffffff;
-        hipLaunchKernelGGL((k_roots_compact<DEG>), dim3((unsigned)cb), dim3(kRootsBlock), 0, s, c,
+        hipLaunchKernelGGL((k_roots_compact<DEG>), dim3((unsigned)blocks), dim3(kRootsBlock), 0, s, c,
                            ri, r, st, ok, N, itmax);
         return hipGetLastError() == hipSuccess ? CAUS_OK : CAUS_ERR_LAUNCH;
     }
@@ -283,8 +296,8 @@ __global__ __launch_bounds__(kRootsBlock) void k_roots_vjp(const cd* __restrict_
                                                            const cd* __restrict__ roots,
                                                            const cd* __restrict__ g,
                                                            cd* __restrict__ gc, int64_t N) {
-    const int64_t i = (int64_t)blockIdx.x * kRootsBlock + threadIdx.x;
-    if (i >= N) return;
+    for (int64_t i = (int64_t)blockIdx.x * kRootsBlock + threadIdx.x; i < N;
+         i += (int64_t)gridDim.x * kRootsBlock) {
     cd p[DEG + 1], acc[DEG + 1];
 #pragma unroll
     for (int k = 0; k <= DEG; ++k) {
@@ -309,11 +322,13 @@ __global__ __launch_bounds__(kRootsBlock) void k_roots_vjp(const cd* __restrict_
     }
 #pragma unroll
     for (int k = 0; k <= DEG; ++k) gc[i * (DEG + 1) + k] = acc[k];
+    }
 }
 
 template <int DEG>
 int roots_vjp_for_degree(hipStream_t s, const cd* c, const cd* r, const cd* g, cd* gc, int64_t N) {
-    const int64_t blocks = (N + kRootsBlock - 1) / kRootsBlock;
+    const int64_t nb = (N + kRootsBlock - 1) / kRootsBlock;
+    const int64_t blocks = nb < kRootsMaxBlocks ? nb : kRootsMaxBlocks;
     hipLaunchKernelGGL((k_roots_vjp<DEG>), dim3((unsigned)blocks), dim3(kRootsBlock), 0, s, c, r, g,
                        gc, N);
     return hipGetLastError() == hipSuccess ? CAUS_OK : CAUS_ERR_LAUNCH;

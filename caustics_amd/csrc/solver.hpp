@@ -228,19 +228,77 @@ __device__ __forceinline__ void horner_comp_value(const cd (&p)[DEG + 1], cd x, 
     h = h + e;
 }
 
-// One Newton step per root with the compensated residual.  For a root that met the
-// stopping test, the step is far below an ulp of the root and is computed to a small
-// fraction of one, so the result is (nearly always) the double nearest the exact root:
-// roots that different iterations (FP64 or CF32, other initial estimates) left a few
-// ulps apart become the same doubles, and decisions taken on them downstream agree.
-// Clustered roots (near a caustic) can still differ.
+// Value and derivative of the polynomial, both by compensated Horner: the derivative
+// recurrence hd <- hd x + h carries its own TwoProd/TwoSum error terms, plus the error
+// of the value it consumes.  At a nearly multiple root p'(z) is far smaller than the
+// rounding error of a plain FP64 derivative, which would leave a Newton or Aberth
+// correction there without a single correct digit.
+template <int DEG>
+__device__ __forceinline__ void horner_comp_vd(const cd (&p)[DEG + 1], cd x, cd& h, cd& hd) {
+#pragma clang fp contract(off)
+    cd e = cd{0, 0}, ed = cd{0, 0};
+    h = p[DEG];
+    hd = cd{0, 0};
+#pragma unroll
+    for (int i = DEG - 1; i >= 0; --i) {
+        cd r, e1, e2, e3, se;
+        two_prod_c(hd, x, r, e1, e2, e3);
+        two_sum_c(r, h, hd, se);
+        const cd td = cd{(e1.re + e2.re) + (e3.re + se.re), (e1.im + e2.im) + (e3.im + se.im)};
+        ed = cd{fma(ed.re, x.re, fma(-ed.im, x.im, e.re + td.re)),
+                fma(ed.re, x.im, fma(ed.im, x.re, e.im + td.im))};
+        two_prod_c(h, x, r, e1, e2, e3);
+        two_sum_c(r, p[i], h, se);
+        const cd t = cd{(e1.re + e2.re) + (e3.re + se.re), (e1.im + e2.im) + (e3.im + se.im)};
+        e = cd{fma(e.re, x.re, fma(-e.im, x.im, t.re)), fma(e.re, x.im, fma(e.im, x.re, t.im))};
+    }
+    h = h + e;
+    hd = hd + ed;
+}
+
+// Canonical roots: Ehrlich-Aberth sweeps with the compensated value and derivative
+// (accurate as if computed in twice the working precision; Aberth sum in FP64).  A root
+// that met the FP64 stopping test lies within a few ulps of an exact root of the FP64
+// polynomial, so one such step takes it to a small fraction of an ulp from it: the
+// result is the double nearest the exact root whatever iteration converged to it (FP64
+// or CF32, other initial estimates or warm starts).  Sweeps repeat while some root
+// still moved by more than 2^-40 of its modulus, which happens only inside clusters of
+// nearly coincident roots (images at a caustic, images next to a lens of very small
+// mass), where the FP64 stopping test accepts iterates far from the roots; there the
+// Aberth term keeps the cluster's roots apart and the compensated derivative keeps the
+// corrections accurate, so they converge to the nearest doubles as well.  The discrete
+// decisions taken downstream (image filter, continuation, segments, stitching) then see
+// the same doubles in both precision forms.
+// The first sweep takes the derivative in plain FP64: away from clusters its relative
+// error only scales a correction of a few ulps, and most roots need no second sweep.  A
+// root whose correction was below 2^-40 of its modulus is at its nearest double after
+// that step (its error was squared), so later sweeps visit only the roots that moved more.
+constexpr int kPolishSweeps = 8;
 template <int DEG>
 __device__ __forceinline__ void polish_roots(const cd (&p)[DEG + 1], cd (&z)[DEG]) {
+    unsigned act = (1u << DEG) - 1u;
+    for (int sweep = 0; sweep < kPolishSweeps && act; ++sweep) {
+        unsigned nxt = 0;
 #pragma unroll
-    for (int j = 0; j < DEG; ++j) {
-        cd h, hd;
-        horner_comp_value<DEG>(p, z[j], h, hd);
-        if (norm2(hd) > 0.0) z[j] = z[j] - cdiv<2>(h, hd);
+        for (int j = 0; j < DEG; ++j) {
+            if ((act >> j) & 1u) {
+                cd h, hd;
+                if (sweep == 0)
+                    horner_comp_value<DEG>(p, z[j], h, hd);
+                else
+                    horner_comp_vd<DEG>(p, z[j], h, hd);
+                const cd den = hd - h * aberth_sum<DEG, 2>(z, j, z[j]);
+                if (norm2(den) > 0.0) {
+                    const cd corr = cdiv<2>(h, den);
+                    const cd zn = z[j] - corr;
+                    if (isfinite(zn.re) && isfinite(zn.im)) {
+                        if (norm2(corr) > 0x1p-80 * norm2(z[j])) nxt |= 1u << j;
+                        z[j] = zn;
+                    }
+                }
+            }
+        }
+        act = nxt;
     }
 }
 
@@ -256,7 +314,7 @@ __device__ __forceinline__ bool comp_step(const cd (&p)[DEG + 1], cd (&z)[DEG], 
     if (az > 1) {
         cd x = crcp<2>(zj);
         horner_comp<DEG, true>(p, x, h, hd, b);
-        double ah = cabs(h);
+        double ah = cabs_safe(h);
         double eb = EA_EPS * ah + (g * b + 2 * EA_EPS * EA_EPS * ah);
         if (ah > 4 * eb) {
             cd S = aberth_sum<DEG>(z, j, zj);
@@ -264,7 +322,7 @@ __device__ __forceinline__ bool comp_step(const cd (&p)[DEG + 1], cd (&z)[DEG], 
             cd num = z2 * h;
             cd den = ((double)DEG * zj) * h - hd - num * S;
             cd corr = cdiv<2>(num, den);
-            if (cabs(corr) > 4 * EA_EPS * az) {
+            if (cabs_safe(corr) > 4 * EA_EPS * az) {
                 z[j] = zj - corr;
                 return false;
             }
@@ -272,12 +330,12 @@ __device__ __forceinline__ bool comp_step(const cd (&p)[DEG + 1], cd (&z)[DEG], 
         return true;
     } else {
         horner_comp<DEG, false>(p, zj, h, hd, b);
-        double ah = cabs(h);
+        double ah = cabs_safe(h);
         double eb = EA_EPS * ah + (g * b + 2 * EA_EPS * EA_EPS * ah);
         if (ah > 4 * eb) {
             cd S = aberth_sum<DEG>(z, j, zj);
             cd corr = cdiv<2>(h, hd - h * S);
-            if (cabs(corr) > 4 * EA_EPS) {
+            if (cabs_safe(corr) > 4 * EA_EPS) {
                 z[j] = zj - corr;
                 return false;
             }
@@ -290,7 +348,32 @@ __device__ __forceinline__ bool comp_step(const cd (&p)[DEG + 1], cd (&z)[DEG], 
 // b = sum alpha'_k |x|^k of the stopping test comes from the bare v_sqrt_f64 (the bound
 // scales a threshold; an error of a few ulps in it moves the test negligibly), the
 // residual h is exact FP64 Horner.
+// Step for root j of a polynomial whose stopping bound tb = eps*b leaves the range in
+// which squares are normal numbers (roots far below or above 1 with the coefficients
+// scaled to O(1)): the test |h| > tb and the correction h/(hd - h S) in power-of-two
+// scaled arithmetic, as thrust::abs (hypot) and thrust's scaled division do it in the
+// reference.  Returns true when the root is converged.
 template <int DEG>
+__device__ __noinline__ bool scaled_step(cd (&z)[DEG], int j, cd h, cd hd, double tb) {
+    const cd x = z[j];
+    if (!(tb > 0.0)) return tb == 0.0 && cmaxabs(h) == 0.0;  // a root at 0 exactly; NaN: not
+    if (!isfinite(tb)) return false;   // |x| beyond the range of doubles: not converged
+    const double s = pow2_inv(tb);
+    const cd hs = cd{h.re * s, h.im * s};
+    const double ts = tb * s;
+    if (!(norm2(hs) > ts * ts)) return true;
+    cd S = cd{0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < DEG; ++i)
+        if (i != j) S = S + crcp_scaled(x - z[i]);
+    z[j] = x - cdiv_scaled(h, hd - h * S);
+    return false;
+}
+
+// GUARD: route polynomials whose squared stopping bound is not a normal number to
+// scaled_step (the public root solver; the lens pipelines' polynomials have O(1)
+// coefficients and roots and skip the two comparisons).
+template <int DEG, bool GUARD = false>
 __device__ __forceinline__ bool fast_step(const cd (&p)[DEG + 1], const double (&al)[DEG + 1],
                                           cd (&z)[DEG], int j) {
     cd x = z[j];
@@ -305,6 +388,14 @@ __device__ __forceinline__ bool fast_step(const cd (&p)[DEG + 1], const double (
         h = cfma(h, x, p[k]);
     }
     const double tb = EA_EPS * b;
+    if (GUARD && !(tb >= 0x1p-480 && tb <= 0x1p480)) {
+        // |x| itself may have underflowed in norm2: recompute b with a scaled modulus
+        const double axs = cabs_safe(x);
+        double bs = al[DEG];
+#pragma unroll
+        for (int k = DEG - 1; k >= 0; --k) bs = fma(bs, axs, al[k]);
+        return scaled_step<DEG>(z, j, h, hd, EA_EPS * bs);
+    }
     if (norm2(h) > tb * tb) {
         cd S = aberth_sum<DEG, 0>(z, j, x);
         z[j] = x - cdiv<2>(h, hd - h * S);
@@ -314,17 +405,17 @@ __device__ __forceinline__ bool fast_step(const cd (&p)[DEG + 1], const double (
 }
 
 // Plain FP64 step for root j with the reference's branch structure (RefCompat form).
-template <int DEG>
+template <int DEG, bool GUARD = false>
 __device__ __forceinline__ bool ref_step(const cd (&p)[DEG + 1], const double (&al)[DEG + 1],
                                          cd (&z)[DEG], int j, int& nrev) {
     cd zj = z[j];
-    double az = cabs(zj);
+    double az = GUARD ? cabs_safe(zj) : cabs(zj);
     double b;
     cd h, hd = cd{0.0, 0.0};
     if (az > 1) {
         ++nrev;
         double ax = rcp(az);
-        cd x = crcp<2>(zj);
+        cd x = GUARD ? crcp_scaled(zj) : crcp<2>(zj);
         b = al[0];
 #pragma unroll
         for (int k = 1; k <= DEG; ++k) b = fma(b, ax, al[k]);
@@ -334,7 +425,7 @@ __device__ __forceinline__ bool ref_step(const cd (&p)[DEG + 1], const double (&
             hd = cfma(hd, x, h);
             h = cfma(h, x, p[k]);
         }
-        if (cabs(h) > EA_EPS * b) {
+        if ((GUARD ? cabs_safe(h) : cabs(h)) > EA_EPS * b) {
             cd S = aberth_sum<DEG>(z, j, zj);
             cd z2 = pow2_explog(zj);
             cd num = z2 * h;
@@ -353,6 +444,7 @@ __device__ __forceinline__ bool ref_step(const cd (&p)[DEG + 1], const double (&
             hd = cfma(hd, zj, h);
             h = cfma(h, zj, p[k]);
         }
+        if (GUARD && !(EA_EPS * b >= 0x1p-480)) return scaled_step<DEG>(z, j, h, hd, EA_EPS * b);
         if (cabs(h) > EA_EPS * b) {
             cd S = aberth_sum<DEG>(z, j, zj);
             z[j] = zj - cdiv<2>(h, hd - h * S);
@@ -362,8 +454,8 @@ __device__ __forceinline__ bool ref_step(const cd (&p)[DEG + 1], const double (&
     }
 }
 
-// Iteration state of one polynomial (registers of one lane).
-template <int DEG, int MODE, bool REFCOMPAT>
+// Iteration state of one polynomial (registers of one lane).  GUARD: see fast_step.
+template <int DEG, int MODE, bool REFCOMPAT, bool GUARD = false>
 struct EAState {
     double al[DEG + 1];  // alpha' weights of the stopping test
     unsigned cx, cy;     // per-root convergence: plain stage, compensated stage
@@ -373,7 +465,7 @@ struct EAState {
     // alpha_i *= 1 + (1+2 sqrt 2) i (:96-99)
     __device__ __forceinline__ void start(const cd (&p)[DEG + 1], cd (&z)[DEG], bool custom_init) {
 #pragma unroll
-        for (int i = 0; i <= DEG; ++i) al[i] = cabs(p[i]);
+        for (int i = 0; i <= DEG; ++i) al[i] = GUARD ? cabs_safe(p[i]) : cabs(p[i]);
         if (!custom_init) init_est<DEG, REFCOMPAT>(al, z);
 #pragma unroll
         for (int i = 0; i <= DEG; ++i) al[i] = al[i] * fma(ALPHA_W, (double)i, 1.0);
@@ -387,7 +479,8 @@ struct EAState {
         for (int j = 0; j < DEG; ++j) {
             if (!((cx >> j) & 1u)) {
                 ++st.it64;
-                bool c = REFCOMPAT ? ref_step<DEG>(p, al, z, j, nrev) : fast_step<DEG>(p, al, z, j);
+                bool c = REFCOMPAT ? ref_step<DEG, GUARD>(p, al, z, j, nrev)
+                                   : fast_step<DEG, GUARD>(p, al, z, j);
                 if (c) cx |= 1u << j;
             } else if (MODE == MODE_FP64_COMP && !((cy >> j) & 1u)) {
                 ++st.itcomp;
@@ -403,10 +496,10 @@ struct EAState {
 // Solve one polynomial held in registers.  Returns true when all roots converged
 // (at the compensated level for MODE_FP64_COMP).  z holds the initial estimates on
 // entry when custom_init is set.
-template <int DEG, int MODE, bool REFCOMPAT>
+template <int DEG, int MODE, bool REFCOMPAT, bool GUARD = false>
 __device__ __forceinline__ bool solve_fp64(const cd (&p)[DEG + 1], cd (&z)[DEG], int itmax,
                                            bool custom_init, SolveStats& st) {
-    EAState<DEG, MODE, REFCOMPAT> S;
+    EAState<DEG, MODE, REFCOMPAT, GUARD> S;
     S.start(p, z, custom_init);
     st.sweeps = 0;
     st.it64 = 0;
@@ -415,6 +508,22 @@ __device__ __forceinline__ bool solve_fp64(const cd (&p)[DEG + 1], cd (&z)[DEG],
     bool done = false;
     for (int it = 0; it < itmax && !done; ++it) done = S.sweep(p, z, st);
     return done;
+}
+
+// Coefficients whose largest component lies outside [2^-256, 2^256] are scaled by a
+// power of two into [1, 2): exact, the roots are unchanged, and the squares of the
+// solver's moduli stay normal numbers (the reference's hypot-based |.| and scaled
+// division never square).  Other polynomials are left bit for bit as given.
+template <int DEG>
+__device__ __forceinline__ void prescale_coeffs(cd (&p)[DEG + 1]) {
+    double m = 0.0;
+#pragma unroll
+    for (int k = 0; k <= DEG; ++k) m = fmax(m, cmaxabs(p[k]));
+    if (m > 0.0 && isfinite(m) && !(m >= 0x1p-256 && m <= 0x1p256)) {
+        const double s = pow2_inv(m);
+#pragma unroll
+        for (int k = 0; k <= DEG; ++k) p[k] = cd{p[k].re * s, p[k].im * s};
+    }
 }
 
 }  // namespace caus

@@ -10,7 +10,8 @@ def match_rows(a, b):
     a = np.asarray(a)
     b = np.asarray(b)
     out = b.copy()
-    same = np.all(np.abs(a - b) <= 1e-8 * (1 + np.abs(a)), axis=1)
+    # rows already in the same order (relative test: roots may be far below 1)
+    same = np.all(np.abs(a - b) <= 1e-8 * np.abs(a), axis=1)
     for i in np.nonzero(~same)[0]:
         d = np.abs(a[i][:, None] - b[i][None, :])
         r, c = linear_sum_assignment(d)
