@@ -16,6 +16,8 @@ CAUS_OK = 0
 CAUS_ERR_ARG = -1
 CAUS_ERR_LAUNCH = -2
 
+LC_NOALLOC_MAX_BYTES = 8 << 20   # CAUS_LC_NOALLOC_MAX_BYTES
+
 MODE_FP64 = 0
 MODE_FP64_COMP = 1
 MODE_CF32 = 2
@@ -43,6 +45,8 @@ SIGNATURES = {
     "caus_lightcurve_workspace_bytes": (ctypes.c_int64, [_i32, _i64, _i64]),
     "caus_lightcurve_point_source": (ctypes.c_int, [_vp, _i32, _vp, _i64, _vp, _i64, _vp, _vp,
                                                     _vp, _vp, _i64, _vp, _i32, _i32]),
+    "caus_lightcurve_point_source_noalloc": (ctypes.c_int, [_vp, _i32, _vp, _i64, _vp, _i64, _vp,
+                                                            _vp, _vp, _vp, _i32, _i32]),
     "caus_extended_workspace_bytes": (ctypes.c_int64, [_i32, _i64, _i32]),
     "caus_mag_extended_source": (ctypes.c_int, [_vp, _i32, _vp, _i64, _vp, _vp, _i64, _vp, _i32,
                                                 _i32, _vp, _i64, _i32, _vp, _vp, _vp, _i64, _i32,

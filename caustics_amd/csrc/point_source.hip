@@ -1,7 +1,16 @@
 // C-ABI entry points of the fused point-source magnification and light-curve kernels.
+#include <atomic>
+
 #include "ps_impl.hpp"
 
 using namespace caus;
+
+// Scratch of caus_lightcurve_point_source_noalloc: the per-(light curve, epoch chunk)
+// partial gradient sums that the caller's workspace holds in
+// caus_lightcurve_point_source.  Every GPU the library runs on holds one copy until the
+// process ends; calls on one GPU share it.
+static constexpr int64_t kLcScratchBytes = CAUS_LC_NOALLOC_MAX_BYTES;
+static __device__ __align__(256) double scratch[kLcScratchBytes / sizeof(double)];
 
 namespace {
 
@@ -102,4 +111,41 @@ extern "C" int caus_lightcurve_point_source(void* stream, int32_t nlenses, const
     hipLaunchKernelGGL(k_lightcurve_finalize, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
                        workspace, g_params, L, nchunk, nq);
     return hipGetLastError() == hipSuccess ? CAUS_OK : CAUS_ERR_LAUNCH;
+}
+
+namespace {
+// device address of `scratch` on the current device, looked up once per device (the
+// host-side name of a __device__ array is not a device address)
+double* lc_scratch_ptr() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    static std::atomic<double*> cached[64];
+    double* p = cached[dev].load(std::memory_order_acquire);
+    if (!p) {
+        void* q = nullptr;
+        if (hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)) != hipSuccess || !q) return nullptr;
+        p = static_cast<double*>(q);
+        cached[dev].store(p, std::memory_order_release);
+    }
+    return p;
+}
+}  // namespace
+
+extern "C" int caus_lightcurve_point_source_noalloc(void* stream, int32_t nlenses,
+                                                    const double* t, int64_t T,
+                                                    const double* lc_params, int64_t L,
+                                                    const double* g_mag, double* mag,
+                                                    double* g_params, uint8_t* status,
+                                                    int32_t itmax, int32_t mode) {
+    if (nlenses < 1 || nlenses > 3 || T < 0 || L < 0) return CAUS_ERR_ARG;
+    double* ws = nullptr;
+    int64_t wsb = 0;
+    if (g_mag && T > 0 && L > 0) {
+        wsb = caus_lightcurve_workspace_bytes(nlenses, T, L);
+        if (wsb < 0 || wsb > kLcScratchBytes) return CAUS_ERR_ARG;
+        ws = lc_scratch_ptr();
+        if (!ws) return CAUS_ERR_LAUNCH;
+    }
+    return caus_lightcurve_point_source(stream, nlenses, t, T, lc_params, L, g_mag, mag, g_params,
+                                        ws, wsb, status, itmax, mode);
 }

@@ -119,6 +119,21 @@ int caus_lightcurve_point_source(void* stream, int32_t nlenses, const double* t,
                                  int64_t workspace_bytes, uint8_t* status, int32_t itmax,
                                  int32_t mode);
 
+/* caus_lightcurve_point_source without a workspace argument: it allocates nothing and
+ * takes no caller memory beyond inputs and outputs.  No counterpart in the reference; it
+ * is for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The per-chunk partial gradient sums
+ * live in a static device array of the library, CAUS_LC_NOALLOC_MAX_BYTES (8 MiB) per
+ * GPU, held until the process ends: batches whose caus_lightcurve_workspace_bytes exceed
+ * it are refused with CAUS_ERR_ARG before anything is enqueued.  Every call on one GPU
+ * shares that array, so calls that overlap on one GPU (concurrent streams, threads) race;
+ * calls ordered on one stream do not.  Forward-only calls (g_mag NULL) use no scratch. */
+#define CAUS_LC_NOALLOC_MAX_BYTES (8 << 20)
+int caus_lightcurve_point_source_noalloc(void* stream, int32_t nlenses, const double* t, int64_t T,
+                                         const double* lc_params, int64_t L, const double* g_mag,
+                                         double* mag, double* g_params, uint8_t* status,
+                                         int32_t itmax, int32_t mode);
+
 /* Hexadecapole magnification and contour gate of mag() (src/caustics/lightcurve.py:168-227):
  * _mag_hexadecapole (src/caustics/multipole.py:208-244) with linear limb darkening u1
  * (mag() itself passes u1 = 0, lightcurve.py:215), _caustics_proximity_test
