@@ -426,15 +426,14 @@ def bench_extended_binary(steps, warmup, n_near=2048, n_far=2048, cpu_s=0.0):
     rho = torch.as_tensor(b["rho"], device="cuda")
     u1 = torch.as_tensor(b["u1"], device="cuda")
     prm = torch.as_tensor(np.stack([b["s"], b["q"]], -1), device="cuda").contiguous()
-    npts_all = np.concatenate([b["npts"], np.full(n_far, 256)])
-    npts = torch.as_tensor(npts_all, device="cuda", dtype=torch.int32)
+    npts_all = np.concatenate([b["npts"], np.full(n_far, 256)]).astype(np.int64)
     gm = torch.ones(S, dtype=torch.float64, device="cuda")
     out = {"workload": f"{S} binary limb-darkened sources (u=0.5), {n_near} near caustics "
                        "with ragged 256-4096 limb points, rest hexadecapole-gated; mag() "
                        "fwd+VJP"}
     res = {}
     for form, mode in (("fp64", _lib.MODE_FP64), ("cf32", _lib.MODE_CF32)):
-        cfg = (2, npts, int(npts_all.max()), True, 100, 2500, mode)
+        cfg = (2, npts_all, int(npts_all.max()), True, 100, 2500, mode)
 
         def step():
             res[form] = _mag_fwd(cfg, w, rho, u1, prm, g=gm)

@@ -76,7 +76,8 @@ extern "C" int caus_mag_extended_source_work(void* stream, int32_t nlenses, cons
                                              uint64_t* work) {
     if (S < 0 || nlenses < 1 || nlenses > 3 || itmax < 0 || params_stride < 0 || u1_stride < 0)
         return CAUS_ERR_ARG;
-    if (mode != CAUS_MODE_FP64 && mode != CAUS_MODE_CF32) return CAUS_ERR_ARG;
+    if (mode != CAUS_MODE_FP64 && mode != CAUS_MODE_FP64_COMP && mode != CAUS_MODE_CF32)
+        return CAUS_ERR_ARG;
     if (npts_max < 4 || ext_mmax(npts_max) > kExtSortMax) return CAUS_ERR_ARG;
     if (limb_darkening && (npts_ld < 2 || npts_ld > 256 || !u1)) return CAUS_ERR_ARG;
     if (S == 0) return CAUS_OK;

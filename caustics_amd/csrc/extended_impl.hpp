@@ -166,35 +166,41 @@ __device__ __forceinline__ uint64_t nn_map(const cd (&a)[DEG], const cd (&b)[DEG
     return conflict ? (p | kConflictBit) : p;
 }
 
-// roots in lexicographic order of (re, im): an odd-even transposition network with
-// compile-time indices (the array stays in registers)
-template <int DEG>
-__device__ __forceinline__ void sort_roots(cd (&z)[DEG]) {
-#pragma unroll
-    for (int pass = 0; pass < DEG; ++pass) {
-#pragma unroll
-        for (int i = pass & 1; i + 1 < DEG; i += 2) {
-            const bool sw = z[i + 1].re < z[i].re || (z[i + 1].re == z[i].re && z[i + 1].im < z[i].im);
-            const cd lo = sw ? z[i + 1] : z[i], hi = sw ? z[i] : z[i + 1];
-            z[i] = lo;
-            z[i + 1] = hi;
-        }
-    }
+// ---- limb sampling ------------------------------------------------------------
+// Initial limb angle k of m (extended_source.py:101-103): numpy/jax linspace(-pi, pi,
+// m, endpoint=False) is start + k * step with the product and the sum rounded
+// separately, then pi - 1e-8 as the last angle; no FMA contraction, so the angles are
+// the same doubles.
+__device__ __forceinline__ double ext_limb_angle(int k, int m) {
+#pragma clang fp contract(off)
+    if (k == m) return 3.14159265358979323846 - 1e-8;
+    const double step = (3.14159265358979323846 - -3.14159265358979323846) / (double)m;
+    const double y = (double)k * step;
+    return y + -3.14159265358979323846;
+}
+// Source-plane limb point w0 + rho e^{i th} (extended_source.py:79-98), each component a
+// product then a sum as the reference's complex arithmetic rounds it.
+__device__ __forceinline__ cd ext_limb_point(cd w0, double rho, double cs, double sn) {
+#pragma clang fp contract(off)
+    const double x = rho * cs, y = rho * sn;
+    return cd{x + w0.re, y + w0.im};
 }
 
 // ---- point solve -----------------------------------------------------------
 // warm = true: z holds initial roots (those of a nearby limb point).  The solve ends
 // with the compensated polish, which takes converged roots to the doubles nearest the
 // exact roots, so the result does not depend on where the iteration started and the
-// rows are put in order afterwards by value (match_points).  canon = true sorts the
-// roots by value: the first limb point's row order seeds the continuation of every row
-// along the limb (greedy matching resolves competing images in row order), so it must
-// not depend on the iteration either.
+// rows are put in order afterwards by value (match_points).  first = true (limb point
+// 0): the row order of this point seeds the continuation of every row along the limb
+// (greedy matching resolves competing images in row order), so it is solved by the
+// reference-compatible FP64 form from the reference's initial estimates
+// (init_est.h:57-102, ehrlich_aberth.h:60-150), which puts the roots in the reference
+// solver's order, in both precision forms.
 template <int NL, int MODE>
 __device__ __forceinline__ bool ext_solve_point(cd w, const LensFrame<NL>& lf,
                                                 cd (&z)[ext_degree<NL>()], uint32_t& flags,
                                                 int itmax, SolveStats* stats = nullptr,
-                                                bool warm = false, bool canon = false) {
+                                                bool warm = false, bool first = false) {
     constexpr int DEG = ext_degree<NL>();
     bool ok = true;
     flags = 0;
@@ -209,8 +215,10 @@ __device__ __forceinline__ bool ext_solve_point(cd w, const LensFrame<NL>& lf,
             if (norm2(z[j]) < 1.0) flags |= 1u << (16 + j);
         }
     } else {
-        ok = solve_lens<NL, MODE, true>(w, lf, z, itmax, warm, stats);
-        if (canon) sort_roots<DEG>(z);
+        if (first)
+            ok = solve_lens_refcompat<NL>(w, lf, z, itmax, stats);
+        else
+            ok = solve_lens<NL, MODE, true>(w, lf, z, itmax, warm, stats);
 #pragma unroll
         for (int j = 0; j < DEG; ++j) {
             ImageTerms<NL> it = image_terms<NL>(z[j], lf);
@@ -1232,11 +1240,11 @@ static __global__ __launch_bounds__(kExtPlanBlock) void k_ext_plan(ExtArgs A) {
         if (s < A.S) {
             int n = A.npts ? A.npts[s] : A.npts_def;
             bool bad = false;
-            if (n < 4 || n > A.npts_def) {
+            if (n != 0 && (n < 4 || n > A.npts_def)) {
                 bad = true;
                 n = n < 4 ? 4 : A.npts_def;
             }
-            B.npts[s] = n;
+            B.npts[s] = n;   // 0: the source is skipped (no limb points, no work)
             B.fail[s] = bad ? 1 : 0;
             ca = (ext_npts_init(n) + kExtPer - 1) / kExtPer;
             cb = ext_n_add(n);
@@ -1272,7 +1280,7 @@ static __global__ __launch_bounds__(kExtPlanBlock) void k_ext_plan(ExtArgs A) {
 // inside one cluster of nearly double images (near a caustic), where the FP64 stopping
 // test accepts both before they separate; the compensated polish that ends every solve
 // continues the Aberth iteration there and takes them to the cluster's distinct roots.
-// Limb point 0 is stored with its roots sorted by value (ext_solve_point, canon).
+// Limb point 0 is solved in the reference's root order (ext_solve_point, first).
 template <int NL, int MODE>
 __global__ __launch_bounds__(kExtFlatBlock)
 __attribute__((amdgpu_waves_per_eu(2)))  // 2 waves/SIMD measured faster than 1
@@ -1299,12 +1307,12 @@ void k_ext_limb_init(ExtArgs A) {
         cd z[DEG];
         SolveStats st{0, 0, 0, 0};
         for (int k = k0; k < k1; ++k) {
-            const double th = (k < M0 - 1) ? -kExtPi + k * (2.0 * kExtPi / (M0 - 1)) : kExtPi - 1e-8;
+            const double th = ext_limb_angle(k, M0 - 1);
             double sn, cs;
             sincos(th, &sn, &cs);
             uint32_t f;
             // the lane's later angles start from the previous angle's roots
-            if (!ext_solve_point<NL, MODE>(w0 + rho * cd{cs, sn}, lf, z, f, A.itmax,
+            if (!ext_solve_point<NL, MODE>(ext_limb_point(w0, rho, cs, sn), lf, z, f, A.itmax,
                                            A.work ? &st : nullptr, k > k0, k == 0))
                 B.fail[s] = 1;
             W.theta[0][k] = th;
@@ -1338,6 +1346,7 @@ __global__ __launch_bounds__(kExtWaveBlock) void k_ext_chain(ExtArgs A, int cur,
     const int tid = threadIdx.x;
     const int64_t s = blockIdx.x;
     const ExtBatch B = ext_batch(A);
+    if (B.npts[s] == 0) return;   // skipped source
     ExtWS W = ext_ws(ext_source_base<DEG>(A, s), DEG, A.mmax);
     if (!final_) {
         const int M = ext_points(B.npts[s], 0);
@@ -1377,6 +1386,7 @@ __global__ __launch_bounds__(NT) void k_ext_select(ExtArgs A, int cur, int round
     const int64_t src = blockIdx.x;
     const ExtBatch B = ext_batch(A);
     const int n = B.npts[src];
+    if (n == 0) return;   // skipped source
     const int M = ext_points(n, round), nadd = ext_n_add(n);
     ExtWS W = ext_ws(ext_source_base<DEG>(A, src), DEG, A.mmax);
     const int nxt = cur ^ 1;
@@ -1479,7 +1489,7 @@ void k_ext_solve_add(ExtArgs A, int cur) {
         for (int r = 0; r < DEG; ++r) z[r] = left[r];   // start from the left neighbour
         uint32_t f;
         SolveStats st{0, 0, 0, 0};
-        if (!ext_solve_point<NL, MODE>(w0 + rho * cd{cs, sn}, lf, z, f, A.itmax,
+        if (!ext_solve_point<NL, MODE>(ext_limb_point(w0, rho, cs, sn), lf, z, f, A.itmax,
                                        A.work ? &st : nullptr, true))
             B.fail[s] = 1;
         if (A.work) ext_count(A.work, st, 1);
@@ -1522,6 +1532,7 @@ __global__ __launch_bounds__(kExtWaveBlock) void k_ext_contours(ExtArgs A, int b
     const int lane = tid & 63;
     const int64_t src = blockIdx.x;
     const ExtBatch B = ext_batch(A);
+    if (B.npts[src] == 0) return;   // skipped source
     const int M = ext_points(B.npts[src], 10);
     ExtWS W = ext_ws(ext_source_base<DEG>(A, src), DEG, A.mmax);
     ExtContours* C = ext_contours<DEG>(A, src);
@@ -1779,7 +1790,12 @@ __global__ __launch_bounds__(kExtWaveBlock) void k_ext_contours(ExtArgs A, int b
                         act.np = np0 + 1;
                         act.t = t0 + (sq.len - 1) + 1;
                         if (ct == 1 || ct == 2) act.par = (ct == 1) ? sq.par : -sq.par;
+                        // the reference zeroes a merged segment (rest[i] = 0): its ends
+                        // then rank by their distance from the origin, not as the
+                        // contour's own ends
                         cand[q].empty = 1;
+                        cand[q].head = cd{0.0, 0.0};
+                        cand[q].tail = cd{0.0, 0.0};
                     }
                     (void)par0;
                     __builtin_amdgcn_wave_barrier();
@@ -1813,6 +1829,16 @@ void k_ext_integrate(ExtArgs A, int buf) {
     const int tid = threadIdx.x;
     const int64_t src = blockIdx.x;
     const ExtBatch B = ext_batch(A);
+    if (B.npts[src] == 0) {   // skipped source: zero magnification and gradient
+        if (tid == 0) {
+            A.mag[src] = 0.0;
+            if (A.status) A.status[src] = 1;
+        }
+        if constexpr (VJP) {
+            if (tid < ND) A.g_out[src * ND + tid] = 0.0;
+        }
+        return;
+    }
     const int M = ext_points(B.npts[src], 10);
     LensFrame<NL> lf;
     cd w0;
@@ -1959,15 +1985,20 @@ int mag_extended_nl(hipStream_t s, const ExtArgs& a, int mode, bool ld, bool vjp
 #define CAUS_DEFINE_EXT(NL)                                                                     \
     template <>                                                                                 \
     int mag_extended_nl<NL>(hipStream_t s, const ExtArgs& a, int mode, bool ld, bool vjp) {     \
-        switch ((mode == MODE_CF32 ? 4 : 0) + (ld ? 2 : 0) + (vjp ? 1 : 0)) {                   \
+        const int m = mode == MODE_FP64 ? 0 : mode == MODE_FP64_COMP ? 1 : 2;                \
+        switch (m * 4 + (ld ? 2 : 0) + (vjp ? 1 : 0)) {                                         \
             case 0: CAUS_EXT_CASE(NL, MODE_FP64, false, false)                                  \
             case 1: CAUS_EXT_CASE(NL, MODE_FP64, false, true)                                   \
             case 2: CAUS_EXT_CASE(NL, MODE_FP64, true, false)                                   \
             case 3: CAUS_EXT_CASE(NL, MODE_FP64, true, true)                                    \
-            case 4: CAUS_EXT_CASE(NL, MODE_CF32, false, false)                                  \
-            case 5: CAUS_EXT_CASE(NL, MODE_CF32, false, true)                                   \
-            case 6: CAUS_EXT_CASE(NL, MODE_CF32, true, false)                                   \
-            case 7: CAUS_EXT_CASE(NL, MODE_CF32, true, true)                                    \
+            case 4: CAUS_EXT_CASE(NL, MODE_FP64_COMP, false, false)                             \
+            case 5: CAUS_EXT_CASE(NL, MODE_FP64_COMP, false, true)                              \
+            case 6: CAUS_EXT_CASE(NL, MODE_FP64_COMP, true, false)                              \
+            case 7: CAUS_EXT_CASE(NL, MODE_FP64_COMP, true, true)                               \
+            case 8: CAUS_EXT_CASE(NL, MODE_CF32, false, false)                                  \
+            case 9: CAUS_EXT_CASE(NL, MODE_CF32, false, true)                                   \
+            case 10: CAUS_EXT_CASE(NL, MODE_CF32, true, false)                                  \
+            case 11: CAUS_EXT_CASE(NL, MODE_CF32, true, true)                                   \
             default:                                                                            \
                 return CAUS_ERR_ARG;                                                            \
         }                                                                                       \

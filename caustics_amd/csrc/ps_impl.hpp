@@ -46,6 +46,25 @@ __device__ __forceinline__ bool solve_lens(cd wl, const LensFrame<NL>& lf, cd (&
     return ok;
 }
 
+// The reference-compatible FP64 solve (reference initial estimates and branch structure:
+// the reference solver's root order) from scratch, then the compensated polish.
+template <int NL>
+__device__ __forceinline__ bool solve_lens_refcompat(cd wl, const LensFrame<NL>& lf,
+                                                     cd (&z)[ps_degree<NL>()], int itmax,
+                                                     SolveStats* stats = nullptr) {
+    constexpr int DEG = ps_degree<NL>();
+    cd p[DEG + 1];
+    lens_poly<NL>(wl, lf, p);
+    SolveStats st;
+    const bool ok = solve_fp64<DEG, MODE_FP64, true>(p, z, itmax, false, st);
+    polish_roots<DEG>(p, z);
+    if (stats) {
+        stats->sweeps += st.sweeps;
+        stats->it64 += st.it64;
+    }
+    return ok;
+}
+
 // Value and tangents of the magnification at lens-frame source position wl.
 // Tangent direction d has source-plane tangent dw[d] and lens-frame tangent lt[d].
 template <int NL, int MODE, int ND>

@@ -6,18 +6,23 @@ per source, the images of the source limb with the reference's 10 rounds of gap
 refinement, chains the image rows along the limb, splits and stitches the image
 segments into closed contours and integrates them (uniform: Green's theorem with the
 trapezoid rule; linear limb darkening: Dominik's P/Q line integrals with Gauss-Legendre
-quadrature), all inside one HIP workgroup per source.  Gradients with respect to the
-source centre, rho, u1 and the lens parameters come from the fused forward+VJP kernel.
+quadrature).  The work runs as one fixed sequence of HIP launches over the whole batch:
+flat launches for the limb solves (one limb point per lane), one wave or workgroup per
+source for continuation, gap selection, stitching and integration.  Gradients with
+respect to the source centre, rho, u1 and the lens parameters come from the fused
+forward+VJP pass.  ``roots_compensated=True`` runs the limb solves with the
+compensated stage (ehrlich_aberth.h:153-292).
 
 Unlike the reference (one scalar source per call) ``w0`` may be an array of sources;
 ``rho``, ``u1``, ``npts_limb`` and the lens parameters broadcast against it, so a
-ragged batch (different ``npts_limb`` per source) is one launch.
+ragged batch (different ``npts_limb`` per source) is one call.  A call synchronises
+nothing with the host, so it can be captured in a HIP graph.
 """
 import numpy as np
 import torch
 
 from . import _lib
-from .point_source import PARAM_NAMES
+from .point_source import PARAM_NAMES, _mode_id
 
 __all__ = ["mag_extended_source", "extended_workspace_bytes", "extended_fwd_vjp"]
 
@@ -36,36 +41,79 @@ def _col(x, n, dtype=torch.float64):
     return t.contiguous()
 
 
-def _run(nlenses, w, rho_t, prm, npts_t, npts_max, ld, u1_t, npts_ld, itmax, mode_id, g=None,
-         max_chunk=None, work=None, streams=2):
+def _host_npts(npts_limb, S):
+    """npts_limb as a host int64 array of length S (or None for one value): the batch
+    is ordered and cut into chunks on the host, from values the caller holds there."""
+    if isinstance(npts_limb, torch.Tensor):
+        raise TypeError("npts_limb must be a host value (int, list or numpy array)")
+    a = np.asarray(npts_limb, dtype=np.int64)
+    if a.ndim == 0:
+        return None
+    a = np.broadcast_to(a.reshape(-1), (S,)) if a.size == 1 else a.reshape(-1)
+    if a.size != S:
+        raise ValueError("npts_limb does not broadcast against w0")
+    return a
+
+
+_LAYOUTS = {}
+
+
+def _ragged_layout(npts_host, S, npts_max, need_npts):
+    """Host-side order of a ragged batch (descending npts_limb, stable), the sorted
+    per-source sizes, and their device copies (order, npts as int32).  The device copies
+    are kept per layout (a few, least recently used first out), so a repeated call, and
+    in particular a call captured in a HIP graph after a warm-up call, copies nothing
+    from the host."""
+    if npts_host is None and not need_npts:
+        return None, None, None
+    key = (S, npts_max, None if npts_host is None else npts_host.tobytes())
+    hit = _LAYOUTS.pop(key, None)
+    if hit is None:
+        if npts_host is not None:
+            eff = np.clip(npts_host, 4, npts_max)
+            order_np = np.argsort(-eff, kind="stable")
+            eff = eff[order_np]
+            sorted_n = np.asarray(npts_host)[order_np]
+            order = (None if np.array_equal(order_np, np.arange(S))
+                     else torch.as_tensor(order_np, device="cuda"))
+        else:
+            eff, order, sorted_n = None, None, np.full(S, npts_max, np.int64)
+        npts_t = torch.as_tensor(np.ascontiguousarray(sorted_n), dtype=torch.int32, device="cuda")
+        hit = (order, eff, npts_t)
+    _LAYOUTS[key] = hit
+    while len(_LAYOUTS) > 8:
+        _LAYOUTS.pop(next(iter(_LAYOUTS)))
+    return hit
+
+
+def _run(nlenses, w, rho_t, prm, npts_host, npts_max, ld, u1_t, npts_ld, itmax, mode_id, g=None,
+         max_chunk=None, work=None, streams=2, skip=None):
     """One or more launch sequences of the contour pipeline, one per chunk of sources.
-    A ragged batch (per-source npts_limb) is processed in descending npts_limb order, so
-    that every launch holds sources of similar cost (no workgroup of a long contour
-    trailing a launch of short ones) and each chunk's workspace is sized by its own
-    largest contour; results return in the caller's order.  With several chunks they
-    alternate over `streams` streams, each with its own workspace, so that one chunk's
-    low-occupancy launches (refinement rounds, continuation, stitching) overlap another
-    chunk's solves and integrals; the caller's stream waits for all of them.  Per-source
-    results do not depend on the batch around them.  `work`, a (5,) int64 CUDA tensor,
-    accumulates the kernels' work counters (FP64, FP32 and compensated root-iterations,
-    limb-point solves, contour points)."""
+    A ragged batch (npts_host, per-source npts_limb held on the host) is processed in
+    descending npts_limb order, so that every launch holds sources of similar cost (no
+    workgroup of a long contour trailing a launch of short ones) and each chunk's
+    workspace is sized by its own largest contour; results return in the caller's order.
+    With several chunks they alternate over `streams` streams, each with its own
+    workspace, so that one chunk's low-occupancy launches (refinement rounds,
+    continuation, stitching) overlap another chunk's solves and integrals; the caller's
+    stream waits for all of them.  `skip`, a (S,) bool CUDA tensor, marks sources to
+    leave out (npts_limb 0 for the kernels: no work, magnification and gradient 0); it
+    is applied on the device, so nothing waits for the host.  Per-source results do not
+    depend on the batch around them.  `work`, a (5,) int64 CUDA tensor, accumulates the
+    kernels' work counters (FP64, FP32 and compensated root-iterations, limb-point
+    solves, contour points)."""
     L = _lib.lib()
     S = w.numel()
     P = 0 if prm is None else prm.shape[1]
     if extended_workspace_bytes(nlenses, 1, npts_max) < 0:
         raise ValueError(f"npts_limb={npts_max} out of range (4..4096)")
-    order = None
-    if npts_t is not None and S > 1:
-        eff = np.clip(npts_t.detach().cpu().numpy().astype(np.int64), 4, npts_max)
-        order_np = np.argsort(-eff, kind="stable")
-        eff = eff[order_np]
-        if not np.array_equal(order_np, np.arange(S)):
-            order = torch.as_tensor(order_np, device="cuda")
-            pick = lambda x: None if x is None else x[order].contiguous()
-            w, rho_t, prm, npts_t, u1_t, g = (pick(w), pick(rho_t), pick(prm), pick(npts_t),
-                                              pick(u1_t), pick(g))
-    else:
-        eff = None
+    order, eff, npts_t = _ragged_layout(npts_host, S, npts_max, skip is not None)
+    if order is not None:
+        pick = lambda x: None if x is None else x[order].contiguous()
+        w, rho_t, prm, u1_t, g, skip = (pick(w), pick(rho_t), pick(prm), pick(u1_t), pick(g),
+                                        pick(skip))
+    if skip is not None:
+        npts_t = torch.where(skip, torch.zeros_like(npts_t), npts_t).contiguous()
     mag = torch.empty(S, dtype=torch.float64, device="cuda")
     status = torch.empty(S, dtype=torch.uint8, device="cuda")
     g_out = torch.empty((S, 4 + P), dtype=torch.float64, device="cuda") if g is not None else None
@@ -119,27 +167,33 @@ def _run(nlenses, w, rho_t, prm, npts_t, npts_max, ld, u1_t, npts_ld, itmax, mod
 
 
 class _Extended(torch.autograd.Function):
+    """Each magnification depends only on its own source's inputs, so the Jacobian is
+    block diagonal: when a gradient is needed the forward runs the fused forward+VJP
+    pass with unit cotangents and keeps dmag_i/dtheta_i (S, 4+P); the backward scales
+    it by the incoming cotangents instead of running the contour pipeline again."""
+
     @staticmethod
     def forward(ctx, cfg, w, rho_t, u1_t, *ps):
-        nlenses, npts_t, npts_max, ld, npts_ld, itmax, mode_id, shapes = cfg
+        nlenses, npts_host, npts_max, ld, npts_ld, itmax, mode_id, shapes = cfg
         prm = torch.stack(list(ps), -1).contiguous() if ps else None
-        mag, status, _ = _run(nlenses, w, rho_t, prm, npts_t, npts_max, ld,
-                              u1_t if ld else None, npts_ld, itmax, mode_id)
-        ctx.cfg = cfg
-        ctx.save_for_backward(w, rho_t, u1_t, *ps)
+        need = any(ctx.needs_input_grad[1:])
+        ones = torch.ones(w.numel(), dtype=torch.float64, device="cuda") if need else None
+        mag, status, jac = _run(nlenses, w, rho_t, prm, npts_host, npts_max, ld,
+                                u1_t if ld else None, npts_ld, itmax, mode_id, g=ones)
+        ctx.nps = len(ps)
+        ctx.ld = ld
+        if need:
+            ctx.save_for_backward(jac)
         ctx.mark_non_differentiable(status)
         return mag, status
 
     @staticmethod
     def backward(ctx, g, _gs):
-        nlenses, npts_t, npts_max, ld, npts_ld, itmax, mode_id, shapes = ctx.cfg
-        w, rho_t, u1_t, *ps = ctx.saved_tensors
-        prm = torch.stack(list(ps), -1).contiguous() if ps else None
-        _, _, go = _run(nlenses, w, rho_t, prm, npts_t, npts_max, ld, u1_t if ld else None,
-                        npts_ld, itmax, mode_id, g=g.contiguous().to(torch.float64))
+        (jac,) = ctx.saved_tensors
+        go = g.to(torch.float64)[:, None] * jac
         gw = torch.complex(go[:, 0], go[:, 1])
-        grads = [gw, go[:, 2], go[:, 3] if ld else torch.zeros_like(go[:, 3])]
-        grads += [go[:, 4 + k] for k in range(len(ps))]
+        grads = [gw, go[:, 2], go[:, 3] if ctx.ld else torch.zeros_like(go[:, 3])]
+        grads += [go[:, 4 + k] for k in range(ctx.nps)]
         return (None, *grads)
 
 
@@ -151,9 +205,7 @@ def mag_extended_source(w0, rho, nlenses=2, npts_limb=150, limb_darkening=False,
     Differentiable in w0, rho, u1 and the lens parameters."""
     if nlenses not in (1, 2, 3):
         raise ValueError("`nlenses` has to be set to be <= 3.")
-    if roots_compensated:
-        raise NotImplementedError("the contour kernels run the FP64 or CF32 solver")
-    mode_id = _lib.MODES[mode or "fp64"]
+    mode_id = _mode_id(roots_compensated, mode)
     w = w0 if isinstance(w0, torch.Tensor) else torch.as_tensor(np.asarray(w0))
     shape = tuple(w.shape)
     w = w.to(device="cuda", dtype=torch.complex128).reshape(-1).contiguous()
@@ -162,10 +214,9 @@ def mag_extended_source(w0, rho, nlenses=2, npts_limb=150, limb_darkening=False,
     rho_t = _col(rho, S)
     u1_t = _col(u1, S)
     ps = [_col(params[k], S) for k in names]
-    npts_arr = np.asarray(npts_limb)
-    npts_max = int(npts_arr.max())
-    npts_t = None if npts_arr.ndim == 0 else _col(npts_arr, S, torch.int32)
-    cfg = (nlenses, npts_t, npts_max, bool(limb_darkening), int(npts_ld), int(roots_itmax),
+    npts_host = _host_npts(npts_limb, S)
+    npts_max = int(np.max(npts_limb))
+    cfg = (nlenses, npts_host, npts_max, bool(limb_darkening), int(npts_ld), int(roots_itmax),
            mode_id, None)
     mag, status = _Extended.apply(cfg, w, rho_t, u1_t, *ps)
     mag = mag.reshape(shape)
@@ -181,9 +232,8 @@ def extended_fwd_vjp(w0, rho, g_mag, nlenses=2, npts_limb=150, limb_darkening=Fa
     S = w.numel()
     names = PARAM_NAMES[nlenses]
     prm = torch.stack([_col(params[k], S) for k in names], -1).contiguous() if names else None
-    npts_arr = np.asarray(npts_limb)
-    npts_t = None if npts_arr.ndim == 0 else _col(npts_arr, S, torch.int32)
-    mag, status, g_out = _run(nlenses, w, _col(rho, S), prm, npts_t, int(npts_arr.max()),
-                              limb_darkening, _col(u1, S) if limb_darkening else None, npts_ld,
-                              roots_itmax, _lib.MODES[mode], g=_col(g_mag, S))
+    mag, status, g_out = _run(nlenses, w, _col(rho, S), prm, _host_npts(npts_limb, S),
+                              int(np.max(npts_limb)), limb_darkening,
+                              _col(u1, S) if limb_darkening else None, npts_ld, roots_itmax,
+                              _lib.MODES[mode], g=_col(g_mag, S))
     return mag, g_out, status

@@ -153,72 +153,62 @@ def _gate(nlenses, w, rho_t, prm, itmax):
 
 
 def _mag_fwd(cfg, w, rho_t, u1_t, prm, g=None, work=None):
-    """Gate, then the contour kernel on the rejected points; with g also the VJP of both
-    branches.  Returns (mag, use, g_out or None)."""
+    """Gate, then the contour pipeline on the points the gate rejects; with g also the
+    VJP of both branches.  The gated points are skipped inside the contour launches
+    (npts_limb 0, set on the device from the gate's output) rather than gathered on the
+    host, so nothing waits for the host.  Returns (mag, use, g_out or None)."""
     from .extended_source import _run
 
-    nlenses, npts_t, npts_max, ld, npts_ld, itmax, mode_id = cfg
-    P = w.numel()
-    NP = 0 if prm is None else prm.shape[1]
+    nlenses, npts_host, npts_max, ld, npts_ld, itmax, mode_id = cfg
     mu, use = _gate(nlenses, w, rho_t, prm, itmax)
-    out = mu.clone()
-    go = torch.zeros((P, 4 + NP), dtype=torch.float64, device="cuda") if g is not None else None
-    rej = torch.nonzero(~use).reshape(-1)
+    m, _, gr = _run(nlenses, w, rho_t, prm, npts_host, npts_max, ld, u1_t if ld else None,
+                    npts_ld, itmax, mode_id, g=g, work=work, skip=use)
+    out = torch.where(use, mu, m)
+    go = None
     if g is not None:
-        acc = torch.nonzero(use).reshape(-1)
-        if acc.numel():
-            _, ga = _hex_vjp(nlenses, w[acc].contiguous(), rho_t[acc].contiguous(), 0.0,
-                             None if prm is None else prm[acc].contiguous(),
-                             g[acc].contiguous(), itmax)
-            ga[:, 3] = 0.0   # the gate's multipoles use no limb darkening
-            go[acc] = ga
-    if rej.numel():
-        sub = lambda x: None if x is None else x[rej].contiguous()
-        m, _, gr = _run(nlenses, sub(w), sub(rho_t), sub(prm), sub(npts_t), npts_max, ld,
-                        sub(u1_t) if ld else None, npts_ld, itmax, mode_id, g=sub(g),
-                        work=work)
-        out[rej] = m
-        if g is not None:
-            go[rej] = gr
+        _, ga = _hex_vjp(nlenses, w, rho_t, 0.0, prm, g, itmax)
+        ga[:, 3] = 0.0   # the gate's multipoles use no limb darkening
+        go = torch.where(use[:, None], ga, gr)
     return out, use, go
 
 
 class _Mag(torch.autograd.Function):
+    """Per-point outputs: the forward keeps the block-diagonal Jacobian (a unit-cotangent
+    VJP of both branches) when a gradient is needed, and the backward scales it."""
+
     @staticmethod
     def forward(ctx, cfg, w, rho_t, u1_t, *ps):
         prm = torch.stack(list(ps), -1).contiguous() if ps else None
-        out, _, _ = _mag_fwd(cfg, w, rho_t, u1_t, prm)
-        ctx.cfg = cfg
-        ctx.save_for_backward(w, rho_t, u1_t, *ps)
+        need = any(ctx.needs_input_grad[1:])
+        ones = torch.ones(w.numel(), dtype=torch.float64, device="cuda") if need else None
+        out, _, jac = _mag_fwd(cfg, w, rho_t, u1_t, prm, ones)
+        ctx.nps = len(ps)
+        if need:
+            ctx.save_for_backward(jac)
         return out
 
     @staticmethod
     def backward(ctx, g):
-        w, rho_t, u1_t, *ps = ctx.saved_tensors
-        prm = torch.stack(list(ps), -1).contiguous() if ps else None
-        _, _, go = _mag_fwd(ctx.cfg, w, rho_t, u1_t, prm, g.contiguous().to(torch.float64))
+        (jac,) = ctx.saved_tensors
+        go = g.to(torch.float64)[:, None] * jac
         grads = [torch.complex(go[:, 0], go[:, 1]), go[:, 2], go[:, 3]]
-        grads += [go[:, 4 + k] for k in range(len(ps))]
+        grads += [go[:, 4 + k] for k in range(ctx.nps)]
         return (None, *grads)
 
 
 def _mag_args(w_points, rho, nlenses, npts_limb, limb_darkening, u1, npts_ld, roots_itmax,
               roots_compensated, mode, params):
-    from .extended_source import _col
+    from .extended_source import _col, _host_npts
 
     if nlenses not in (1, 2, 3):
         raise ValueError("`nlenses` has to be set to be <= 3.")
-    if roots_compensated:
-        raise NotImplementedError("the contour kernels run the FP64 or CF32 solver")
     w = w_points if isinstance(w_points, torch.Tensor) else torch.as_tensor(np.asarray(w_points))
     shape = tuple(w.shape)
     w = w.to(device="cuda", dtype=torch.complex128).reshape(-1).contiguous()
     P = w.numel()
     names = PARAM_NAMES[nlenses]
-    npts_arr = np.asarray(npts_limb)
-    npts_t = None if npts_arr.ndim == 0 else _col(npts_arr, P, torch.int32)
-    cfg = (nlenses, npts_t, int(npts_arr.max()), bool(limb_darkening), int(npts_ld),
-           int(roots_itmax), _lib.MODES[mode or "fp64"])
+    cfg = (nlenses, _host_npts(npts_limb, P), int(np.max(npts_limb)), bool(limb_darkening),
+           int(npts_ld), int(roots_itmax), _mode_id(roots_compensated, mode))
     return cfg, shape, w, _col(rho, P), _col(u1, P), [_col(params[k], P) for k in names]
 
 
@@ -237,12 +227,12 @@ def mag(w_points, rho, nlenses=2, npts_limb=200, limb_darkening=False, u1=0.0, n
 
 
 def mag_fwd_vjp(w_points, rho, g_mag, nlenses=2, npts_limb=200, limb_darkening=False, u1=0.0,
-                npts_ld=100, roots_itmax=2500, mode=None, **params):
+                npts_ld=100, roots_itmax=2500, roots_compensated=False, mode=None, **params):
     """Fused forward + VJP of mag(): (mag (P,), g_out (P, 4+n_params), use_hex (P,)), g_out
     columns Re w, Im w, rho, u1 and the lens parameters."""
     cfg, shape, w, rho_t, u1_t, ps = _mag_args(w_points, rho, nlenses, npts_limb,
                                                limb_darkening, u1, npts_ld, roots_itmax,
-                                               False, mode, params)
+                                               roots_compensated, mode, params)
     from .extended_source import _col
 
     prm = torch.stack(ps, -1).contiguous() if ps else None

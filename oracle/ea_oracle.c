@@ -368,3 +368,64 @@ int ora_num_threads(void) {
     return 1;
 #endif
 }
+
+/* ---------------- canonical polish (no counterpart in the reference) ----------------
+ * The same canonicalisation the product applies to every contour limb solve: Ehrlich-
+ * Aberth sweeps whose residual and derivative come from compensated (double-double)
+ * Horner, run while some root still moves by more than 2^-40 of its modulus (at most 8
+ * sweeps; the first sweep takes a plain derivative).  Converged roots end at the doubles
+ * nearest the exact roots of the given FP64 polynomial, which makes image decisions taken
+ * on them (the 1e-6 lens-equation filter) reproducible where the reference's backward-
+ * error stopping test leaves ill-conditioned roots far from them.  Written from the
+ * method (Graillat-Langlois-Louvet compensated Horner), not from the product's code. */
+static void horner_comp_vd(const cplx *p, cplx x, int deg, int deriv_comp, cplx *h, cplx *hd) {
+    cplx e = mk(0, 0), ed = mk(0, 0), res, e1, e2, e3, se;
+    *h = p[deg];
+    *hd = mk(0, 0);
+    for (int i = deg - 1; i >= 0; --i) {
+        if (deriv_comp) {
+            two_prod_cmplx(*hd, x, &res, &e1, &e2, &e3);
+            two_sum_cmplx(res, *h, hd, &se);
+            ed = cadd(cadd(cmul(ed, x), e), mk((e1.re + e2.re) + (e3.re + se.re),
+                                                (e1.im + e2.im) + (e3.im + se.im)));
+        } else {
+            *hd = cadd(cmul(*hd, x), *h);
+        }
+        two_prod_cmplx(*h, x, &res, &e1, &e2, &e3);
+        two_sum_cmplx(res, p[i], h, &se);
+        e = cadd(cmul(e, x), mk((e1.re + e2.re) + (e3.re + se.re), (e1.im + e2.im) + (e3.im + se.im)));
+    }
+    *h = cadd(*h, e);
+    *hd = cadd(*hd, ed);
+}
+
+void ora_polish(int deg, const cplx *p, cplx *z) {
+    unsigned act = (deg >= 32) ? 0xFFFFFFFFu : ((1u << deg) - 1u);
+    for (int sweep = 0; sweep < 8 && act; ++sweep) {
+        unsigned nxt = 0;
+        for (int j = 0; j < deg; ++j) {
+            if (!((act >> j) & 1u)) continue;
+            cplx h, hd;
+            horner_comp_vd(p, z[j], deg, sweep > 0, &h, &hd);
+            cplx den = csub(hd, cmul(h, aberth_sum(z, deg, j)));
+            if (den.re == 0 && den.im == 0) continue;
+            cplx corr = cdiv(h, den);
+            cplx zn = csub(z[j], corr);
+            if (!isfinite(zn.re) || !isfinite(zn.im)) continue;
+            double nc = corr.re * corr.re + corr.im * corr.im;
+            double nz = z[j].re * z[j].re + z[j].im * z[j].im;
+            if (nc > 0x1p-80 * nz) nxt |= 1u << j;
+            z[j] = zn;
+        }
+        act = nxt;
+    }
+}
+
+void ora_polish_batch(int64_t N, int deg, const double *coeffs, double *roots, int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 64)
+#endif
+    for (int64_t i = 0; i < N; i++)
+        ora_polish(deg, (const cplx *)(coeffs + 2 * i * (deg + 1)), (cplx *)(roots + 2 * i * deg));
+}

@@ -85,8 +85,9 @@ def dslice2(x, start):
 # ---------------------------------------------------------------------------
 # limb images
 # ---------------------------------------------------------------------------
-def _solve_points(w0, rho, theta, nlenses, lp):
-    z, mask = lens.images_point_source(rho * np.exp(1j * theta) + w0, nlenses, **lp)
+def _solve_points(w0, rho, theta, nlenses, lp, perturb=0.0, seed=0, polish=False):
+    z, mask = lens.images_point_source(rho * np.exp(1j * theta) + w0, nlenses, perturb=perturb,
+                                       seed=seed, polish=polish, **lp)
     par = np.sign(lens.lens_eq_det_jac(z, nlenses, **lp))
     return z, mask, par
 
@@ -108,25 +109,36 @@ def limb_thetas_initial(npts):
     return np.append(th, np.pi - 1e-8)
 
 
-def images_of_source_limb(w0, rho, nlenses, npts, lp, niter=LIMB_NITER, theta=None):
+def images_of_source_limb(w0, rho, nlenses, npts, lp, niter=LIMB_NITER, theta=None,
+                          perturb=0.0, seed=0, polish=False):
     """theta given: images at exactly these limb angles (sorted, no refinement) with the
     same identity continuation -- the discretisation of an earlier call, frozen, so
-    that finite differences see only the smooth dependence on the inputs."""
+    that finite differences see only the smooth dependence on the inputs.
+    perturb > 0: every limb polynomial's coefficients are multiplied by
+    (1 + perturb (u + i v)), u, v uniform in [-1, 1] (lens.images_point_source), so the
+    spread of results over seeds measures what the FP64 route determines at all.
+    polish=True ends every limb solve with the canonical compensated polish (ea.py):
+    the roots become the doubles nearest the exact roots of each FP64 polynomial, the
+    canonicalisation the product applies; the reference's plain stopping test leaves
+    ill-conditioned roots (clusters near low-mass lenses, near caustics) further away,
+    far enough to move an image across the 1e-6 filter."""
     if theta is not None:
-        z, mask, par = _solve_points(w0, rho, np.asarray(theta), nlenses, lp)
+        z, mask, par = _solve_points(w0, rho, np.asarray(theta), nlenses, lp, perturb, seed,
+                                     polish)
         z, mask, par = chain_permute(z, mask, par)
         return z, mask, par, np.asarray(theta)
     theta = limb_thetas_initial(npts)
-    z, mask, par = _solve_points(w0, rho, theta, nlenses, lp)
+    z, mask, par = _solve_points(w0, rho, theta, nlenses, lp, perturb, seed, polish)
     z, mask, par = chain_permute(z, mask, par)
     n = int(int(0.5 * npts) / niter)
-    for _ in range(niter):
+    for it in range(niter):
         dz = np.abs(z[:, 1:] - z[:, :-1])
         dz = np.where(mask[:, 1:] | mask[:, :-1], dz, 0.0)
         dmax = dz.max(axis=0)
         idcs = np.argsort(dmax, kind="stable")[::-1][:n]
         th_new = 0.5 * (theta[idcs] + theta[idcs + 1])
-        zn, mn, pn = _solve_points(w0, rho, th_new, nlenses, lp)
+        zn, mn, pn = _solve_points(w0, rho, th_new, nlenses, lp, perturb, seed + 1 + it,
+                                   polish)
         for c, k in enumerate(idcs):
             perm = match_points(z[:, k], zn[:, c])
             zn[:, c], mn[:, c], pn[:, c] = zn[perm, c], mn[perm, c], pn[perm, c]
@@ -358,11 +370,38 @@ def integrate_ld(z, tidx, w0, rho, u1, nlenses, lp, npts=100):
 # ---------------------------------------------------------------------------
 # magnification
 # ---------------------------------------------------------------------------
+def mag_from_limb(z, mask, par, w0, rho, nlenses, lp, limb_darkening=False, u1=0.0,
+                  npts_ld=100):
+    """Segments, stitching and integrals (extended_source.py:156-901 after
+    _images_of_source_limb) for given limb images: z, mask, par (n, N) in limb order,
+    rows continued along the limb; w0 in the lens frame."""
+    if limb_darkening:
+        integ = lambda c, t: integrate_ld(c, t, w0, rho, u1, nlenses, lp, npts_ld)
+    else:
+        integ = integrate_unif
+    if nlenses == 1:
+        c, cp = contours_from_closed_segments(np.moveaxis(np.stack([z, par]), 0, 1))
+        I = np.array([integ(c[k], z.shape[1] - 1) for k in range(2)])
+        return np.abs(np.sum(I * cp)) / (np.pi * rho**2)
+    seg_c, seg_o, all_closed = get_segments(z, mask, par, nlenses)
+    c1, p1 = contours_from_closed_segments(seg_c)
+    I1 = np.array([integ(c1[k], c1.shape[1] - 1) for k in range(c1.shape[0])]) * p1
+    I2 = np.zeros(3)
+    if not all_closed:
+        c2, p2 = contours_from_open_segments(seg_o)
+        I2 = np.array([integ(c2[k], last_nonzero(c2[k].real)) for k in range(3)]) * p2
+    return np.abs(I1.sum() + I2.sum()) / (np.pi * rho**2)
+
+
 def mag_extended_source(w0, rho, nlenses=2, npts_limb=150, limb_darkening=False, u1=0.0,
-                        npts_ld=100, return_parts=False, theta=None, **params):
+                        npts_ld=100, return_parts=False, theta=None, perturb=0.0, seed=0,
+                        polish=False, **params):
     lp, x_cm = lens.lens_params(nlenses, **params)
     w0 = complex(w0) + x_cm
-    z, mask, par, theta = images_of_source_limb(w0, rho, nlenses, npts_limb, lp, theta=theta)
+    z, mask, par, theta = images_of_source_limb(w0, rho, nlenses, npts_limb, lp, theta=theta,
+                                                perturb=perturb, seed=seed, polish=polish)
+    if not return_parts:
+        return mag_from_limb(z, mask, par, w0, rho, nlenses, lp, limb_darkening, u1, npts_ld)
     if limb_darkening:
         integ = lambda c, t: integrate_ld(c, t, w0, rho, u1, nlenses, lp, npts_ld)
     else:

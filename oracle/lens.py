@@ -173,7 +173,7 @@ def lens_eq_det_jac(z, nlenses=2, reference_single_lens=False, **lp):
 
 
 def images_point_source(w, nlenses=2, itmax=2500, compensated=False, init_mode=0, perturb=0.0,
-                        seed=0, **lp):
+                        seed=0, polish=False, **lp):
     """(n, *w.shape) images and real-image mask (point_source.py:1652-1709).
 
     perturb > 0 multiplies every coefficient by (1 + perturb*(u + i v)), u, v uniform in
@@ -190,7 +190,8 @@ def images_point_source(w, nlenses=2, itmax=2500, compensated=False, init_mode=0
     if perturb:
         g = np.random.default_rng(seed)
         c = c * (1 + perturb * (g.uniform(-1, 1, c.shape) + 1j * g.uniform(-1, 1, c.shape)))
-    z = ea.poly_roots(c, itmax=itmax, compensated=compensated, init_mode=init_mode)
+    z = ea.poly_roots(c, itmax=itmax, compensated=compensated, init_mode=init_mode,
+                      polish=polish)
     z = np.moveaxis(z, -1, 0)
     mask = np.abs(lens_eq(z, nlenses, **lp) - w) < 1e-6
     return z, mask

@@ -153,7 +153,7 @@ def test_chunks_and_streams_do_not_change_results(gpu):
     prm = torch.as_tensor(np.tile([0.9, 0.2], (S, 1)), device="cuda").contiguous()
     u1 = torch.full((S,), 0.5, dtype=torch.float64, device="cuda")
     gm = torch.as_tensor(g.normal(size=S), device="cuda")
-    npts = torch.as_tensor(g.choice([64, 128, 256, 600], S), dtype=torch.int32, device="cuda")
+    npts = g.choice([64, 128, 256, 600], S).astype(np.int64)
     args = (2, w, rho, prm, npts, 600, True, u1, 40, 2500, _lib.MODE_FP64)
     one = _run(*args, g=gm, streams=1, max_chunk=S)
     many = _run(*args, g=gm, streams=2, max_chunk=257)
@@ -279,3 +279,70 @@ def test_mag_lightcurve_gate_and_contours(gpu):
     for i in np.nonzero(~use)[0]:
         mo[i] = E.mag_extended_source(w[i], rho, nlenses=2, npts_limb=200, s=s, q=q)
     compare(m, mo, "mag() with gate")
+
+
+def test_compensated_limb_solves(gpu):
+    """roots_compensated=True (extended_source.py:750,839) runs the limb solves with the
+    compensated stage; both forms end at the same canonical roots, so the contours and
+    magnifications agree with the FP64 form."""
+    prm = dict(s=0.9, q=0.2)
+    rho = 1e-2
+    w = caustic_points(rho, npts=6, **prm)
+    kw = dict(nlenses=2, npts_limb=256, limb_darkening=True, u1=0.5, npts_ld=100, **prm)
+    m64 = mag_extended_source(w, rho, **kw).cpu().numpy()
+    mc = mag_extended_source(w, rho, roots_compensated=True, **kw).cpu().numpy()
+    e = np.abs(mc - m64) / m64
+    print(f"compensated vs plain limb solves: max rel {e.max():.2e}, identical {np.mean(mc == m64):.3f}")
+    assert e.max() <= 1e-12
+    from caustics_amd import mag
+    mm = mag(w, rho, roots_compensated=True, **kw).cpu().numpy()
+    assert np.all(np.isfinite(mm))
+
+
+def test_skipped_sources_cost_nothing_and_leave_others_unchanged(gpu):
+    """Sources marked to skip (npts_limb 0 for the kernels, set on the device) get
+    magnification and gradient 0; every other source gives the same bits as without
+    them."""
+    from caustics_amd import _lib
+    from caustics_amd.extended_source import _run
+
+    rho = 1e-2
+    w = torch.as_tensor(caustic_points(rho, npts=8, s=0.9, q=0.2), device="cuda")
+    S = w.numel()
+    rho_t = torch.full((S,), rho, dtype=torch.float64, device="cuda")
+    prm = torch.as_tensor(np.tile([0.9, 0.2], (S, 1)), device="cuda").contiguous()
+    u1 = torch.full((S,), 0.5, dtype=torch.float64, device="cuda")
+    g = torch.ones(S, dtype=torch.float64, device="cuda")
+    npts = np.random.default_rng(3).choice([128, 256, 300], S).astype(np.int64)
+    skip = torch.as_tensor(np.arange(S) % 3 == 1, device="cuda")
+    args = (2, w, rho_t, prm, npts, 300, True, u1, 50, 2500, _lib.MODE_FP64)
+    m0, st0, g0 = _run(*args, g=g)
+    m1, st1, g1 = _run(*args, g=g, skip=skip)
+    keep = ~skip
+    assert torch.equal(m1[keep], m0[keep]) and torch.equal(g1[keep], g0[keep])
+    assert torch.all(m1[skip] == 0) and torch.all(g1[skip] == 0) and torch.all(st1 == 1)
+
+
+def test_mag_autograd_matches_fused(gpu):
+    """mag()'s backward reuses the Jacobian of its forward pass (per-point outputs): the
+    gradients equal the fused forward+VJP of both branches."""
+    from caustics_amd import mag
+    from caustics_amd.lightcurve import mag_fwd_vjp
+
+    s, q, rho = 0.9, 0.2, 1e-2
+    w = caustic_points(rho, npts=6, s=s, q=q)
+    w = np.concatenate([w, w + 0.15])
+    wt = torch.as_tensor(w, device="cuda").requires_grad_(True)
+    st = torch.tensor(s, dtype=torch.float64, device="cuda", requires_grad=True)
+    rt = torch.tensor(rho, dtype=torch.float64, device="cuda", requires_grad=True)
+    gm = torch.as_tensor(np.random.default_rng(2).normal(size=len(w)), device="cuda")
+    m = mag(wt, rt, nlenses=2, npts_limb=256, limb_darkening=True, u1=0.5, s=st, q=q)
+    (m * gm).sum().backward()
+    mf, go, use = mag_fwd_vjp(w, rho, gm, nlenses=2, npts_limb=256, limb_darkening=True, u1=0.5,
+                              s=s, q=q)
+    assert torch.equal(m.detach(), mf)
+    np.testing.assert_allclose(wt.grad.cpu().numpy(), (go[:, 0] + 1j * go[:, 1]).cpu().numpy(),
+                               rtol=1e-13, atol=0)
+    np.testing.assert_allclose(st.grad.item(), go[:, 4].sum().item(), rtol=1e-12)
+    np.testing.assert_allclose(rt.grad.item(), go[:, 2].sum().item(), rtol=1e-12)
+    assert use.any() and (~use).any()

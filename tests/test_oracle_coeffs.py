@@ -56,3 +56,25 @@ def test_forward_map_invariant(nlenses):
     pz = (c * z[:, None] ** k).sum(-1)
     scale = (np.abs(c) * np.abs(z[:, None]) ** k).sum(-1)
     assert np.max(np.abs(pz) / scale) < 1e-13
+
+
+def test_critical_curve_coefficients_match_notebook():
+    """Critical-curve polynomials (point_source.py:1482-1534) against the notebook's
+    printed sympy outputs (cells 19 and 22) evaluated at 50 digits: the oracle's
+    restatement and the package's torch builder (evaluated here on CPU tensors)."""
+    import torch
+
+    from caustics_amd.point_source import _critical_coeffs
+
+    f = np.load(FIX)
+    phi, a, e1, e2, r3 = f["phi"], f["a"], f["e1"], f["e2"], f["r3"]
+    c2 = np.array([lens.poly_coeffs_critical(phi[i], 2, a=a[i], e1=e1[i]) for i in range(64)])
+    c3 = np.array([lens.poly_coeffs_critical(phi[i], 3, a=a[i], e1=e1[i], e2=e2[i], r3=r3[i])
+                   for i in range(64)])
+    assert rel(c2, f["crit2"]) < 1e-14
+    assert rel(c3, f["crit3"]) < 1e-14
+    t = lambda x: torch.as_tensor(x)[:, None]
+    p2 = _critical_coeffs(t(phi), 2, t(a), t(e1))[:, 0].numpy()
+    p3 = _critical_coeffs(t(phi), 3, t(a), t(e1), t(e2), t(r3))[:, 0].numpy()
+    assert rel(p2, f["crit2"]) < 1e-14
+    assert rel(p3, f["crit3"]) < 1e-14
