@@ -1,9 +1,9 @@
 """Extended-source contour kernels vs the oracle (oracle/extended.py restating
 extended_source.py / integrate.py) and vs closed forms.
 
-Bars: within 1e-10 relative of the oracle for nearly every source (a discrete choice
-near a threshold may differ: those stay within the discretisation error, 1e-3), and
-within the reference tests' own rtol 1e-3 of the single-lens closed forms."""
+Bars: every source within 1e-10 relative of the oracle (the largest deviation observed
+is 5e-11), and within the reference tests' own rtol 1e-3 of the single-lens closed
+forms."""
 import numpy as np
 import pytest
 import torch
@@ -31,8 +31,7 @@ def single_lens_uniform(u, rho):
 def compare(m, mo, name):
     e = np.abs(m - mo) / np.abs(mo)
     print(f"{name}: n={len(m)} max rel err {e.max():.3e}, frac<=1e-10 {(e <= 1e-10).mean():.3f}")
-    assert (e <= 1e-10).mean() >= 0.9
-    assert e.max() < 1e-3
+    assert e.max() <= 1e-10
 
 
 @pytest.mark.parametrize("rho", [1.0, 0.1, 0.01])
@@ -115,18 +114,19 @@ def test_cf32_form_matches_fp64(gpu, nlenses):
     e = np.abs(m32 - m64) / m64
     eg = np.abs(g32 - g64) / np.maximum(np.abs(g64), 1e-3 * m64[:, None])
     print(f"cf32 vs fp64 nl={nlenses}: mag max rel {e.max():.2e}, grad max rel {eg.max():.2e}")
-    # both forms end each limb solve with the same compensated Newton step, which takes
-    # converged roots to the doubles nearest the exact roots: the contours coincide
+    # both forms end each limb solve with the same canonical polish, which takes converged
+    # roots to the doubles nearest the exact roots: the contours coincide (observed:
+    # bit-identical)
     assert e.max() <= 1e-8
-    assert eg.max() <= 1e-6
-    # a larger batch across caustic crossings: clustered images may still differ
+    assert eg.max() <= 1e-8
+    # a larger batch across caustic crossings, clustered images included
     w2 = caustic_points(rho, nlenses=nlenses, npts=40, **prm)
     w2 = w2 + rho * 0.7 * np.exp(1j * np.linspace(0, 20, len(w2)))
     m64, _, _ = extended_fwd_vjp(w2, rho, np.ones(len(w2)), mode="fp64", **kw)
     m32, _, _ = extended_fwd_vjp(w2, rho, np.ones(len(w2)), mode="cf32", **kw)
     e2 = ((m32 - m64).abs() / m64).cpu().numpy()
     print(f"  {len(w2)} sources: frac <= 1e-8 {np.mean(e2 <= 1e-8):.4f}")
-    assert np.mean(e2 <= 1e-8) >= 0.98
+    assert e2.max() <= 1e-8
 
 
 def test_long_contours_match_oracle(gpu):
@@ -217,7 +217,7 @@ def test_extended_vjp_vs_finite_differences(gpu, ld):
         err = np.abs(g[:, col] - fd) / np.maximum(np.abs(fd), 1e-3 * mag)
         print(f"{name}: stable {stable.mean():.2f}, max rel err vs FD {err[stable].max():.2e}")
         assert stable.mean() >= 0.75
-        assert np.all(err[stable] < 1e-3)
+        assert np.all(err[stable] < 2e-4)   # observed at most 2.1e-5
 
 
 def test_extended_grad_reference_case(gpu):
@@ -246,7 +246,7 @@ def test_extended_grad_reference_case(gpu):
     jac = np.array([g[4], g[5], g[2], g[3]])
     err = np.abs((jac - jfd) / jfd)
     print("reference gradient case: rel err", err)
-    assert np.all(err < 1e-3)
+    assert np.all(err < 2e-4)   # observed at most 2.3e-5
 
 
 def test_extended_autograd_matches_fused(gpu):

@@ -28,8 +28,9 @@ def test_gate_matches_oracle_near_caustics(gpu, q):
             for sd in (1, 2)]
     ok = agree_within_spread(mu, mo, pert)
     print(f"agreement within the oracle's own coefficient spread {ok.mean():.4f}")
-    assert ok.mean() >= 0.999
-    assert (use == uo).mean() >= 0.995
+    # observed: every value within the spread, every decision the oracle's
+    assert ok.all()
+    assert (use == uo).all()
 
 
 def test_hexadecapole_u1_and_single_lens(gpu):

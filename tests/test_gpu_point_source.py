@@ -37,7 +37,7 @@ def test_binary_c1_matches_oracle(gpu, mode, tol):
     mo = lens.mag_point_source(w, nlenses=2, s=0.9, q=0.1)
     e = rel(m, mo)
     print(f"[{mode}] max rel err {e.max():.3e}  p99 {np.quantile(e, 0.99):.3e}")
-    assert np.mean(e <= tol) >= 0.9999
+    assert e.max() <= tol   # observed at most 2.2e-12 in every form
     assert np.all(m >= 1.0 - 1e-12)
 
 
@@ -114,7 +114,7 @@ def test_gradients_vs_oracle_and_fd(gpu, nlenses):
     scale = np.maximum(np.abs(ref), 1e-3 * np.abs(mo)[:, None])
     err = (np.abs(ours - ref) / scale)[well]
     print(f"grad max scaled err {err.max():.3e} on {well.sum()} well-conditioned points")
-    assert np.quantile(err, 0.999) < 1e-6
+    assert err.max() < 1e-5   # observed at most 7.8e-7 (triple), 2.2e-9 (binary)
     # central differences on s and Re(w) (h = 1e-6)
     h = 1e-6
     for k, key in ((2, "s"), (0, "w")):
@@ -145,9 +145,9 @@ def test_gradients_vs_oracle_and_fd(gpu, nlenses):
         # (h and 2h disagree) say nothing about the derivative
         stable = np.abs(fd - fd2) <= 1e-5 * np.maximum(np.abs(fd), mo)
         e = (np.abs(fd - ours[:, k]) / np.maximum(np.abs(fd), 1e-2 * mo))[well & stable]
-        print(f"FD {key}: stable {np.mean(stable[well]):.3f}, p99 rel err {np.quantile(e, 0.99):.2e}")
+        print(f"FD {key}: stable {np.mean(stable[well]):.3f}, max rel err {e.max():.2e}")
         assert np.mean(stable[well]) > 0.9
-        assert np.quantile(e, 0.99) < 1e-4
+        assert e.max() < 1e-3
 
 
 def test_lightcurve_fwd_and_vjp(gpu):
@@ -182,20 +182,28 @@ def test_lightcurve_fwd_and_vjp(gpu):
     (m2 * torch.as_tensor(g, device="cuda")).sum().backward()
     ga = np.stack([cols[k].grad.cpu().numpy() for k in ("t0", "u0", "tE", "alpha", "s", "q")], -1)
     np.testing.assert_allclose(ga, gp, rtol=1e-12, atol=1e-12)
-    # central differences (h = 1e-6) of sum(g * mag) per light curve, on the oracle
-    h = 1e-6
-    for k, key in enumerate(("t0", "u0", "tE", "alpha", "s", "q")):
+    # central differences of sum(g * mag) per light curve, on the oracle, at h = 1e-6 and
+    # 2h: a curve counts where the two agree to 1e-4 (an image appearing at a caustic
+    # between the two evaluations spoils the quotient)
+    def tot(pq):
+        wt = W.trajectory(t, pq["t0"], pq["u0"], pq["tE"], pq["alpha"])
+        return np.array([(g[l] * lens.mag_point_source(wt[l], nlenses=2, s=pq["s"][l],
+                                                      q=pq["q"][l])).sum() for l in range(L)])
+
+    def fd(key, h):
         pp, pm = dict(prm), dict(prm)
         pp[key] = prm[key] + h
         pm[key] = prm[key] - h
-        def tot(pq):
-            wt = W.trajectory(t, pq["t0"], pq["u0"], pq["tE"], pq["alpha"])
-            return np.array([(g[l] * lens.mag_point_source(wt[l], nlenses=2, s=pq["s"][l],
-                                                          q=pq["q"][l])).sum() for l in range(L)])
-        fd = (tot(pp) - tot(pm)) / (2 * h)
-        e = np.abs(fd - gp[:, k]) / np.maximum(np.abs(fd), 1e-3 * np.abs(g * mo).sum(1))
-        print(key, np.max(e))
-        assert np.median(e) < 1e-5
+        return (tot(pp) - tot(pm)) / (2 * h)
+
+    for k, key in enumerate(("t0", "u0", "tE", "alpha", "s", "q")):
+        f1, f2 = fd(key, 1e-6), fd(key, 2e-6)
+        scale = np.maximum(np.abs(f1), 1e-3 * np.abs(g * mo).sum(1))
+        stable = np.abs(f1 - f2) <= 1e-4 * scale
+        e = (np.abs(f1 - gp[:, k]) / scale)[stable]
+        print(f"{key}: stable {stable.mean():.2f}, max rel err {e.max():.2e}")
+        assert stable.mean() >= 0.5
+        assert e.max() < 1e-3
 
 
 @pytest.mark.parametrize("nlenses", [2, 3])
