@@ -7,10 +7,22 @@ psi ~ U[0,2pi), q3 log-uniform [1e-4,1e-1]; sources uniform in |w| <= 2), coeffi
 resident in HBM, FP64 Ehrlich-Aberth.  value = polynomials solved per second.
 
 Also reported in the same JSON line: the compensated-FP32 (CF32) form of the same
-solve, per-form error against the FP64 CPU oracle, the roofline of the dominant
-kernel (algorithmic FP64 flops per launch / HIP-event launch time vs the MI355X FP64
-vector peak), and the CPU baseline (the reference's own CPU op, compiled from its
-sources, one core, on a bounded sample).
+solve with its own roofline, per-form error against the FP64 CPU oracle, the roofline of
+the dominant kernel (algorithmic FP64 flops per launch / HIP-event launch time vs the
+MI355X FP64 vector peak), and the CPU baseline (the reference's own CPU op, compiled
+from its sources, on a bounded sample, one core and every core of the process).
+
+Under "workloads": the other hot-path workloads, each timed on the GPU with the
+algorithmic flops of its dominant kernels (forward and reverse mode) against the vector
+peak, its error against the oracle on a sample, and the oracle's own rate on the host
+(one core, and every core through spawned single-thread worker processes):
+  C1 10^4 binary point-source magnifications;
+  C3 256 binary light curves x 2000 epochs, forward + VJP (FP64 and CF32);
+  C4 4096 limb-darkened binary extended sources, forward + VJP;
+  C5 1024 triple-lens limb-darkened light curves x 1500 epochs, hexadecapole gate and
+     contour integration with 256 limb points, forward + VJP (FP64 and CF32);
+  ll the marginalised likelihood with its gradient; tr the annual-parallax trajectory.
+Every timed region starts after an untimed clock ramp.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -27,6 +39,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector, dense
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector (packed), dense
 HBM_PEAK_GBS = 8000.0
 
 
@@ -39,6 +52,24 @@ def flops_per_root_iter(n):
 
 def flops_comp_root_iter(n):
     return 227 * n + 33
+
+
+def flops_cf32_comp_root_iter(n):
+    """FP32 flops of one root-iteration of the CF32 form's compensated FP32 sweep: per
+    Horner step four TwoProd (2 each) and four TwoSum (6 each) plus the error
+    recurrence (8), then the Aberth sum and correction as in the plain step."""
+    return 48 * n + 10 * (n - 1) + 21
+
+
+# Gauss-Legendre node of a limb-darkened line integral (integrate.py:29-121), counted by
+# the same rule from the kernel's operations: the lens equation with one shared
+# reciprocal, |D|, the profile and the quadrature accumulation (forward), and with the
+# reverse mode also the squared v_j, K, the two spatial derivatives, the parameter
+# adjoints of every lens and the four running sums (adjoint).
+NODE_FLOPS_FWD = {1: 34, 2: 48, 3: 62}
+NODE_FLOPS_VJP = {1: 100, 2: 138, 3: 177}
+# implicit-derivative adjoint of one contour point (image_terms, det J, c, per-lens terms)
+IMAGE_ADJ_FLOPS = {1: 55, 2: 95, 3: 132}
 
 
 def make_inputs(n5, n10):
@@ -59,6 +90,9 @@ def main():
     ap.add_argument("--n5", type=int, default=1 << 20)
     ap.add_argument("--n10", type=int, default=1 << 18)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--ramp-seconds", type=float, default=3.0,
+                    help="untimed sustained work before any timing, so that every timed "
+                         "region runs at steady clocks")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--roots-only", action="store_true")
     ap.add_argument("--workloads", default="c1,c3,c4,c5,ll,tr",
@@ -126,6 +160,17 @@ def main():
     # slowest polynomials) overlaps the other's work.  Each kernel's own duration, for the
     # roofline, is timed separately with HIP events around single launches on one stream.
     s_a, s_b = _lib.side_streams(2)   # the library's pool (default priority)
+    # untimed clock ramp: the device leaves its idle clocks only after a second or two of
+    # sustained work, so a short warmup alone would time a partly ramped device
+    t0 = time.perf_counter()
+    nramp = 0
+    while time.perf_counter() - t0 < args.ramp_seconds:
+        for nl in (2, 3):
+            launch(nl, _lib.MODE_FP64, _lib.INIT_COMPLEX)
+        torch.cuda.synchronize()
+        nramp += 1
+    print(f"bench: clock ramp {nramp} steps, {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+          flush=True)
     results = {}
     for name, (mode, init) in forms.items():
         steps = args.steps if name == "fp64" else max(3, args.steps // 4)
@@ -188,6 +233,22 @@ def main():
     w = main_r["work"][dom]
     flops = float(w[1]) * flops_per_root_iter(n) + float(w[2]) * flops_comp_root_iter(n)
     achieved = flops / (main_r["kernel_ms"][dom] * 1e-3) / 1e12
+    # CF32 form: FP32 stage-A root-iterations and compensated FP32 sweeps against the
+    # FP32 vector peak, its FP64 finishing iterations against the FP64 peak; frac is the
+    # sum of the two time shares (the fraction of the launch the VALU would need at peak)
+    r32 = results["cf32"]
+    w32 = r32["work"][dom]
+    f32 = float(w32[3]) * flops_per_root_iter(n) + float(w32[2]) * flops_cf32_comp_root_iter(n)
+    f64 = float(w32[1]) * flops_per_root_iter(n)
+    t32 = r32["kernel_ms"][dom] * 1e-3
+    r32["roofline"] = {
+        "bound": "valu", "kernel": f"k_roots_compact<{n}> CF32 form", "kernel_ms": t32 * 1e3,
+        "fp32_flops_per_launch": f32, "fp64_flops_per_launch": f64,
+        "achieved_fp32": f32 / t32 / 1e12, "achieved_fp64": f64 / t32 / 1e12,
+        "peak_fp32": FP32_PEAK_TFLOPS, "peak_fp64": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "frac": (f32 / (FP32_PEAK_TFLOPS * 1e12) + f64 / (FP64_PEAK_TFLOPS * 1e12)) / t32,
+        "note": "the FP32 peak counts packed FP32 (two lanes per instruction), which this "
+                "build does not emit; without it the FP32 ceiling equals the FP64 one"}
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
@@ -201,8 +262,10 @@ def main():
     if not args.roots_only:
         sel = set(args.workloads.split(","))
         for key, name, fn in (
-                ("c1", "point_source_1e4", lambda: bench_point_source(args.steps, 2)),
-                ("c3", "lightcurves_256x2000_fwd_vjp", lambda: bench_lightcurves(20, 5)),
+                ("c1", "point_source_1e4",
+                 lambda: bench_point_source(args.steps, 2, cpu_s=0 if args.no_cpu else args.cpu_seconds)),
+                ("c3", "lightcurves_256x2000_fwd_vjp",
+                 lambda: bench_lightcurves(20, 5, cpu_s=0 if args.no_cpu else args.cpu_seconds)),
                 ("c4", "extended_binary_ld_4096_fwd_vjp",
                  lambda: bench_extended_binary(5, 2, cpu_s=0 if args.no_cpu else args.cpu_seconds)),
                 ("c5", "triple_extended_lightcurves_fwd_vjp",
@@ -276,7 +339,7 @@ def _timed(fn, steps, warmup, name="", budget_s=60.0):
     return dt
 
 
-def bench_point_source(steps, warmup):
+def bench_point_source(steps, warmup, cpu_s=0.0):
     """10^4 binary point-source magnifications, s=0.9, q=0.1, w uniform in [-1.5,1.5]^2."""
     import torch
 
@@ -294,11 +357,22 @@ def bench_point_source(steps, warmup):
         _lib.check(L.caus_mag_point_source(_lib.stream_ptr(), 2, _lib.ptr(w), n, _lib.ptr(prm), 0,
                                            _lib.ptr(mag), None, 2500, _lib.MODE_FP64), "ps")
     dt = _timed(run, steps * 10, warmup, "C1 point source")
-    return {"workload": "10^4 binary point-source magnifications (s=0.9, q=0.1), forward, FP64",
-            "mag_evals_per_s": n / dt, "ms": dt * 1e3}
+    out = {"workload": "10^4 binary point-source magnifications (s=0.9, q=0.1), forward, FP64",
+           "mag_evals_per_s": n / dt, "ms": dt * 1e3}
+    mo = _oracle_point_source((w.cpu().numpy(), dict(s=0.9, q=0.1)))
+    e = np.abs(mag.cpu().numpy() - mo) / mo
+    out["vs_oracle"] = {"max_rel_err": float(e.max()), "p99_rel_err": float(np.quantile(e, 0.99)),
+                        "frac_le_1e-10": float(np.mean(e <= 1e-10))}
+    if cpu_s > 0:
+        wc = w.cpu().numpy()
+        chunks = [(wc[i:i + 2000], dict(s=0.9, q=0.1)) for i in range(0, n, 2000)] * 400
+        out["cpu_baseline"] = cpu_rates(_oracle_point_source, chunks, cpu_s,
+                                        "point-source magnifications of the same 10^4 positions",
+                                        "mag evals/s", per_item=2000)
+    return out
 
 
-def bench_lightcurves(steps, warmup, host_check=True):
+def bench_lightcurves(steps, warmup, host_check=True, cpu_s=0.0):
     """256 binary point-source light curves x 2000 epochs, fused forward + VJP wrt
     (t0, u0, tE, alpha, s, q); FP64 and CF32 forms; error of CF32 vs FP64."""
     import torch
@@ -344,12 +418,25 @@ def bench_lightcurves(steps, warmup, host_check=True):
         _, _, st = solve_with_stats(c, mode=name, init="complex")
         sw = st.sum(0).cpu().numpy().astype(np.float64)
         work = [sw[1], sw[3], sw[2], float(L_ * T), 0.0]
-        out[name]["roofline"] = _roofline_entry(contour_flops(work, 2, False),
-                                                out[name]["ms"] * 1e-3)
+        f64, f32 = contour_flops(work, 2, False)
+        f64 += pointsource_vjp_flops(L_ * T, 2, 6)
+        out[name]["roofline"] = _roofline_entry((f64, f32), out[name]["ms"] * 1e-3)
         out[name]["roofline"]["rule"] = ("root-iterations x 160 + per epoch 1e3 (coefficients) "
-                                         "+ 5 x 65 (lens equation, det J); VJP not counted")
+                                         "+ 5 x 65 (lens equation, det J) + VJP 5 roots x 6 "
+                                         "directions x 50")
     m64, g64 = mags["fp64"]
     m32, g32 = mags["cf32"]
+    wtraj = W.trajectory(tt[0], prm["t0"], prm["u0"], prm["tE"], prm["alpha"])
+    curves = [(wtraj[l], {"s": float(prm["s"][l]), "q": float(prm["q"][l])}) for l in range(L_)]
+    mo = np.stack([_oracle_point_source(cv) for cv in curves[:32]])
+    e = np.abs(m64[:32].cpu().numpy() - mo) / mo
+    out["fp64"]["vs_oracle"] = {"curves": 32, "max_rel_err": float(e.max()),
+                                "p99_rel_err": float(np.quantile(e, 0.99)),
+                                "frac_le_1e-10": float(np.mean(e <= 1e-10))}
+    if cpu_s > 0:
+        out["cpu_baseline"] = cpu_rates(_oracle_point_source, curves * 8, cpu_s,
+                                        "light-curve epochs of the same workload, forward only",
+                                        "mag evals/s (forward only)", per_item=T)
     rel = ((m32 - m64).abs() / m64.abs()).cpu().numpy()
     out["cf32"]["max_rel_err_vs_fp64"] = float(rel.max())
     out["cf32"]["p99_rel_err_vs_fp64"] = float(np.quantile(rel, 0.99))
@@ -360,20 +447,33 @@ def bench_lightcurves(steps, warmup, host_check=True):
 FP32_PEAK_TFLOPS = 157.3
 
 
-def contour_flops(work, nlenses, ld, npts_ld=100):
+def contour_flops(work, nlenses, ld, npts_ld=100, vjp=False):
     """Algorithmic flops of a contour run from its work counters (FP64 root-iterations,
     FP32 root-iterations, compensated root-iterations, limb-point solves, contour
     points), by the counting rule of the root solver plus, per limb-point solve, the
     coefficient build (about 10^3 flops binary, 10^4 triple) and n x (lens equation 25 +
-    det J 40), and per limb-darkened contour point 2 npts_ld (lens equation 25 +
-    brightness 15).  The reverse mode is not counted.  Returns (FP64 flops, FP32 flops)."""
+    det J 40), and per limb-darkened contour point 2 npts_ld Gauss-Legendre nodes
+    (NODE_FLOPS_FWD, or NODE_FLOPS_VJP with the reverse mode, which also adds the image
+    adjoint of every contour point).  The compensated polish that ends every limb solve
+    is this implementation's addition and is not counted.  Returns (FP64, FP32 flops)."""
     n = nlenses * nlenses + 1
     it64, it32, itc, solves, points = (float(x) for x in work)
     f64 = it64 * flops_per_root_iter(n) + itc * flops_comp_root_iter(n)
     f64 += solves * ((1e3 if nlenses == 2 else 1e4) + n * 65)
-    f64 += points * (2 * npts_ld * 40 if ld else 4)
+    node = (NODE_FLOPS_VJP if vjp else NODE_FLOPS_FWD)[nlenses]
+    f64 += points * (2 * npts_ld * node if ld else 4)
+    if vjp:
+        f64 += points * IMAGE_ADJ_FLOPS[nlenses]
     f32 = it32 * flops_per_root_iter(n)
     return f64, f32
+
+
+def pointsource_vjp_flops(epochs, nlenses, ndir):
+    """Reverse-mode flops of the fused point-source VJP per the counting rule: for each
+    of the n roots and each of ndir directions, the implicit image tangent
+    dz = (u - K conj u)/det (about 30) and the tangent of 1/|det J| (about 20)."""
+    n = nlenses * nlenses + 1
+    return epochs * n * ndir * 50.0
 
 
 def _roofline_entry(flops, dt):
@@ -383,29 +483,87 @@ def _roofline_entry(flops, dt):
             "achieved_fp32_tflops": f32 / dt / 1e12, "frac": busy}
 
 
-def cpu_extended_baseline(items, budget_s, what):
-    """The oracle's mag_extended_source (NumPy restatement + the C root oracle) on one
-    thread, over `items` (keyword dicts) in order until budget_s is spent.  Forward only:
-    the oracle has no reverse mode, so this is a lower bound on the CPU cost of the
-    fwd+VJP evaluations the GPU figure counts."""
-    import ctypes
-
-    from oracle import extended as OE
+# ---- CPU baselines: the oracle (NumPy restatement + the C root oracle) on the host ----
+def host_cores():
+    """(cores used, how they were counted): every CPU of this process's affinity mask,
+    unless a cgroup CPU quota (cgroup v2 cpu.max) grants fewer CPUs' worth of time."""
+    n = len(os.sched_getaffinity(0))
     try:
-        ctypes.CDLL("libgomp.so.1").omp_set_num_threads(1)   # the C oracle's solves
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if quota != "max":
+            q = int(-(-int(quota) // int(period)))
+            if 0 < q < n:
+                return q, f"cgroup cpu.max quota {q} of {n} affinity CPUs"
+    except (OSError, ValueError):
+        pass
+    return n, f"{n} affinity CPUs"
+
+
+def _pool_init(root):
+    if root not in sys.path:
+        sys.path.insert(0, root)
+    try:
+        ctypes.CDLL("libgomp.so.1").omp_set_num_threads(1)   # one thread per worker
     except OSError:
         pass
-    n = 0
-    t0 = time.perf_counter()
-    for kw in items:
-        OE.mag_extended_source(**kw)
+
+
+def _pool_noop(x):
+    return x
+
+
+def _oracle_extended(kw):
+    from oracle import extended as OE
+    return OE.mag_extended_source(**kw)
+
+
+def _oracle_extended_pair(kw):
+    """(oracle as the reference computes it, oracle with the canonical root polish)"""
+    from oracle import extended as OE
+    return OE.mag_extended_source(**kw), OE.mag_extended_source(polish=True, **kw)
+
+
+def _oracle_point_source(args):
+    from oracle import lens
+    w, prm = args
+    return lens.mag_point_source(w, 2, **prm)
+
+
+def cpu_rates(fn, items, budget_s, what, unit, per_item=1):
+    """Evaluations per second of fn over items on the host: one core (this process, one
+    OpenMP thread) and every core this process may use (os.sched_getaffinity; spawned
+    worker processes of one thread each), each bounded by budget_s of work."""
+    import multiprocessing as mp
+
+    items = list(items)
+    _pool_init(ROOT)
+    n, t0 = 0, time.perf_counter()
+    for it in items:
+        fn(it)
         n += 1
         if time.perf_counter() - t0 > budget_s:
             break
-    dt = time.perf_counter() - t0
-    print(f"bench: CPU oracle {what}: {n} evals in {dt:.1f} s", file=sys.stderr, flush=True)
-    return {"value": n / dt, "unit": "mag evals/s (forward only)", "cores": 1, "kind": "port",
-            "sample": f"{n} {what}, forward only, {dt:.1f} s"}
+    dt1 = time.perf_counter() - t0
+    one = {"value": n * per_item / dt1, "unit": unit, "cores": 1, "kind": "port",
+           "sample": f"{n * per_item} {what}, {dt1:.1f} s"}
+    ncores, how = host_cores()
+    ctx = mp.get_context("spawn")   # fresh interpreters: the workers never touch the GPU
+    with ctx.Pool(ncores, initializer=_pool_init, initargs=(ROOT,)) as pool:
+        pool.map(_pool_noop, range(4 * ncores))   # workers up before the clock starts
+        n, t0 = 0, time.perf_counter()
+        for _ in pool.imap_unordered(fn, items, chunksize=1):
+            n += 1
+            if time.perf_counter() - t0 > budget_s:
+                break
+        dt = time.perf_counter() - t0
+        pool.terminate()
+    allc = {"value": n * per_item / dt, "unit": unit, "cores": ncores, "kind": "port",
+            "sample": f"{n * per_item} {what}, {dt:.1f} s, {ncores} worker processes "
+                      f"({how})"}
+    print(f"bench: CPU {what}: 1 core {one['value']:.4g}, {ncores} cores {allc['value']:.4g} "
+          f"{unit}", file=sys.stderr, flush=True)
+    one["all_cores"] = allc
+    return one
 
 
 def bench_extended_binary(steps, warmup, n_near=2048, n_far=2048, cpu_s=0.0):
@@ -448,7 +606,7 @@ def bench_extended_binary(steps, warmup, n_near=2048, n_far=2048, cpu_s=0.0):
                      "work": {"root_iterations_fp64": int(work[0]),
                               "root_iterations_fp32": int(work[1]),
                               "limb_solves": int(work[3]), "contour_points": int(work[4])},
-                     "roofline": _roofline_entry(contour_flops(work, 2, True), dt),
+                     "roofline": _roofline_entry(contour_flops(work, 2, True, vjp=True), dt),
                      "grad_finite": bool(torch.isfinite(go).all().item())}
     m64, m32 = res["fp64"][0], res["cf32"][0]
     e = ((m32 - m64).abs() / m64.abs()).cpu().numpy()
@@ -463,8 +621,9 @@ def bench_extended_binary(steps, warmup, n_near=2048, n_far=2048, cpu_s=0.0):
                       npts_limb=int(b["npts"][i]), limb_darkening=True, u1=float(b["u1"][i]),
                       npts_ld=100, s=float(b["s"][i]), q=float(b["q"][i]))
                  for i in range(n_near))
-        out["cpu_baseline"] = cpu_extended_baseline(
-            items, cpu_s, "near-caustic C4 sources (their ragged npts_limb)")
+        out["cpu_baseline"] = cpu_rates(_oracle_extended, list(items), cpu_s,
+                                        "near-caustic C4 sources (their ragged npts_limb), "
+                                        "forward only", "mag evals/s (forward only)")
     return out
 
 
@@ -514,7 +673,7 @@ def bench_triple(steps, warmup, n_sets=1024, n_epochs=1500, npts=256, budget_s=4
         work = work.cpu().numpy() * (S / (8 * n_epochs))
         out[form] = {"mag_evals_per_s_fwd_vjp": S / dt, "ms": dt * 1e3,
                      "unconverged": int((res[form][1] == 0).sum().item()),
-                     "roofline": _roofline_entry(contour_flops(work, 3, True), dt),
+                     "roofline": _roofline_entry(contour_flops(work, 3, True, vjp=True), dt),
                      "roofline_note": "work counters from the first 8 parameter sets, scaled"}
     m64, m32 = res["fp64"][0], res["cf32"][0]
     e = ((m32 - m64).abs() / m64.abs()).cpu().numpy()
@@ -529,8 +688,35 @@ def bench_triple(steps, warmup, n_sets=1024, n_epochs=1500, npts=256, budget_s=4
                       s=float(b["s"][k]), q=float(b["q"][k]), q3=float(b["q3"][k]),
                       r3=float(b["r3"][k]), psi=float(b["psi"][k]))
                  for k in range(n_sets) for e in range(0, n_epochs, 25))
-        out["cpu_baseline"] = cpu_extended_baseline(
-            items, cpu_s, "C5 epochs (every 25th epoch of the light curves in order)")
+        out["cpu_baseline"] = cpu_rates(_oracle_extended, list(items)[:20000], cpu_s,
+                                        "C5 epochs (every 25th epoch of the light curves in "
+                                        "order), forward only", "mag evals/s (forward only)")
+        # error of the FP64 GPU form against the oracle on a sample around and away from
+        # the caustic crossings: the reference's route as such, and with the canonical
+        # root polish that the GPU path applies (where the reference's plain stopping
+        # test leaves ill-conditioned images off by enough to flip the 1e-6 filter)
+        rows = [(kk, e) for kk in range(min(k, 32)) for e in (300, 744, 749, 753, 1200)]
+        samp = [dict(w0=complex(b["w"][kk][e]), rho=float(b["rho"][kk]), nlenses=3,
+                     npts_limb=npts, limb_darkening=True, u1=float(b["u1"][kk]), npts_ld=100,
+                     s=float(b["s"][kk]), q=float(b["q"][kk]), q3=float(b["q3"][kk]),
+                     r3=float(b["r3"][kk]), psi=float(b["psi"][kk])) for kk, e in rows]
+        import multiprocessing as mp
+        with mp.get_context("spawn").Pool(host_cores()[0], initializer=_pool_init,
+                                          initargs=(ROOT,)) as pool:
+            pairs = np.array(pool.map(_oracle_extended_pair, samp, chunksize=1))
+        gi = np.array([kk * n_epochs + e for kk, e in rows])
+        g64 = m64.cpu().numpy()[gi]
+        e_p = np.abs(g64 - pairs[:, 0]) / pairs[:, 0]
+        e_c = np.abs(g64 - pairs[:, 1]) / pairs[:, 1]
+        out["fp64"]["vs_oracle"] = {
+            "epochs": len(rows), "sample": "epochs 300, 744, 749, 753, 1200 of the first "
+                                           f"{min(k, 32)} light curves (crossing near 749.5)",
+            "frac_le_1e-10": float(np.mean(e_p <= 1e-10)),
+            "frac_le_1e-8": float(np.mean(e_p <= 1e-8)),
+            "max_rel_err": float(e_p.max()), "p99_rel_err": float(np.quantile(e_p, 0.99)),
+            "polished_oracle": {"frac_le_1e-10": float(np.mean(e_c <= 1e-10)),
+                                "frac_le_1e-8": float(np.mean(e_c <= 1e-8)),
+                                "max_rel_err": float(e_c.max())}}
     return out
 
 
@@ -716,7 +902,7 @@ def cpu_baseline(host, seconds):
            "sample": f"{total_n} polynomials ({off} degree-5 + {off // 4} degree-10) of the "
                      f"same workload, FP64, plain Ehrlich-Aberth, {total_t:.1f} s"}
     # the C restatement with OpenMP over polynomials on every core this process may use
-    ncores = min(len(os.sched_getaffinity(0)), 16)
+    ncores, how = host_cores()
     n5 = min(host[2].shape[0], max(chunk5, int(out["value"] * seconds * ncores * 0.4) // 5 * 4))
     c5, c10 = host[2][:n5], host[3][:n5 // 4]
     ea.roots_asc(c5[:1024], nthreads=ncores)
@@ -726,7 +912,7 @@ def cpu_baseline(host, seconds):
     dt = time.perf_counter() - t0
     out["all_cores"] = {"value": (n5 + n5 // 4) / dt, "unit": "polys/s", "cores": ncores,
                         "kind": "port", "sample": f"{n5} degree-5 + {n5 // 4} degree-10, "
-                                                  f"{dt:.1f} s"}
+                                                  f"{dt:.1f} s, {how}"}
     return out
 
 

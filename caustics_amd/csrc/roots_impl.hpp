@@ -94,7 +94,7 @@ __global__ __launch_bounds__(kRootsBlock) void k_roots_persistent(
             if (done || st.sweeps >= itmax) {
 #pragma unroll
                 for (int k = 0; k < DEG; ++k) roots[i * DEG + k] = z[k];
-                if (status) status[i] = done ? 1 : 0;
+                if (status) status[i] = (done && roots_finite<DEG>(z)) ? 1 : 0;
                 if (stats) {
                     stats[4 * i + 0] = st.sweeps;
                     stats[4 * i + 1] = st.it64;
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(kRootsBlock) void k_roots_compact(
             if (done || st.sweeps >= itmax) {
 #pragma unroll
                 for (int k = 0; k < DEG; ++k) roots[i * DEG + (int)((S.perm >> (4 * k)) & 0xF)] = z[k];
-                if (status) status[i] = done ? 1 : 0;
+                if (status) status[i] = (done && roots_finite<DEG>(z)) ? 1 : 0;
                 if (stats) {
                     stats[4 * i + 0] = st.sweeps;
                     stats[4 * i + 1] = st.it64;

@@ -354,9 +354,12 @@ __device__ __forceinline__ bool comp_step(const cd (&p)[DEG + 1], cd (&z)[DEG], 
 // scaled arithmetic, as thrust::abs (hypot) and thrust's scaled division do it in the
 // reference.  Returns true when the root is converged.
 template <int DEG>
-__device__ __noinline__ bool scaled_step(cd (&z)[DEG], int j, cd h, cd hd, double tb) {
+__device__ __forceinline__ bool scaled_step(cd (&z)[DEG], int j, cd h, cd hd, double tb) {
     const cd x = z[j];
-    if (!(tb > 0.0)) return tb == 0.0 && cmaxabs(h) == 0.0;  // a root at 0 exactly; NaN: not
+    // tb == 0: a root at 0 exactly.  tb NaN: the root itself is NaN and no step can mend
+    // it, so it stops iterating (as the plain test's false comparison stops it) and the
+    // polynomial is reported unconverged by roots_finite.
+    if (!(tb > 0.0)) return tb == 0.0 ? cmaxabs(h) == 0.0 : true;
     if (!isfinite(tb)) return false;   // |x| beyond the range of doubles: not converged
     const double s = pow2_inv(tb);
     const cd hs = cd{h.re * s, h.im * s};
@@ -493,9 +496,19 @@ struct EAState {
     }
 };
 
+// Every root finite: a NaN or infinite root passes the stopping test (its comparisons
+// are false) but is no solution, so a solve counts as converged only with this.
+template <int DEG>
+__device__ __forceinline__ bool roots_finite(const cd (&z)[DEG]) {
+    bool f = true;
+#pragma unroll
+    for (int k = 0; k < DEG; ++k) f = f && isfinite(z[k].re) && isfinite(z[k].im);
+    return f;
+}
+
 // Solve one polynomial held in registers.  Returns true when all roots converged
-// (at the compensated level for MODE_FP64_COMP).  z holds the initial estimates on
-// entry when custom_init is set.
+// (at the compensated level for MODE_FP64_COMP) to finite values.  z holds the
+// initial estimates on entry when custom_init is set.
 template <int DEG, int MODE, bool REFCOMPAT, bool GUARD = false>
 __device__ __forceinline__ bool solve_fp64(const cd (&p)[DEG + 1], cd (&z)[DEG], int itmax,
                                            bool custom_init, SolveStats& st) {
@@ -507,7 +520,7 @@ __device__ __forceinline__ bool solve_fp64(const cd (&p)[DEG + 1], cd (&z)[DEG],
     st.it32 = 0;
     bool done = false;
     for (int it = 0; it < itmax && !done; ++it) done = S.sweep(p, z, st);
-    return done;
+    return done && roots_finite<DEG>(z);
 }
 
 // Coefficients whose largest component lies outside [2^-256, 2^256] are scaled by a

@@ -263,3 +263,21 @@ def test_cf32_nearly_double_roots_regression(gpu):
     assert np.isfinite(z32).all() and st32.cpu().numpy().all()
     ok, _, _ = roots_close(z32, z64, c)
     assert ok.all()
+
+
+def test_cf32_nan_iterates_stop_at_once(gpu):
+    """Three degree-10 polynomials of the seeded triple-lens batch (W.batched_polys(3,
+    2^18, seed=3), rows 81278, 102236, 248030) whose CF32 FP64 stage meets a NaN Aberth
+    sum.  A NaN root must stop iterating at once and hand the polynomial to the FP64
+    fallback, not run the stage's 2000 sweeps (one such lane held its wave ~7 ms)."""
+    lf, w, _ = W.batched_polys(3, 1 << 18, seed=3)
+    rows = np.array([81278, 102236, 248030])
+    c = W.coeffs_product(w[rows], lf["a"][rows], lf["e1"][rows], lf["e2"][rows], lf["r3"][rows], 3)
+    z64, st64, _ = solve_with_stats(c, mode="fp64", init="complex")
+    z32, st32, stats = solve_with_stats(c, mode="cf32", init="complex")
+    s = stats.cpu().numpy()
+    print("CF32 stats (sweeps, FP64 iterations, compensated, FP32):", s.tolist())
+    assert st32.cpu().numpy().all() and np.isfinite(z32.cpu().numpy()).all()
+    assert (s[:, 1] < 1000).all()   # observed with the stuck stage: ~20,100
+    ok, _, _ = roots_close(z32.cpu().numpy(), z64.cpu().numpy(), c)
+    assert ok.all()
