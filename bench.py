@@ -70,6 +70,12 @@ NODE_FLOPS_FWD = {1: 34, 2: 48, 3: 62}
 NODE_FLOPS_VJP = {1: 100, 2: 138, 3: 177}
 # implicit-derivative adjoint of one contour point (image_terms, det J, c, per-lens terms)
 IMAGE_ADJ_FLOPS = {1: 55, 2: 95, 3: 132}
+# lens polynomial of one source position (lens.hpp lens_poly: the products of the lens
+# factors with products by an exact 1 and sums with an exact 0 not counted)
+COEFF_FLOPS = {1: 10, 2: 223, 3: 1016}
+# per image: 1/(conj z - conj r_j) (8), G and K terms (4 + 10) per lens, then z - G,
+# |z - G - w|^2 against the filter, det J = 1 - |K|^2
+IMAGE_FLOPS = {1: 33, 2: 55, 3: 77}
 
 
 def make_inputs(n5, n10):
@@ -404,9 +410,11 @@ def bench_lightcurves(steps, warmup, host_check=True, cpu_s=0.0):
         torch.cuda.synchronize()
         mags[name] = (mag.clone(), gp.clone())
         out[name] = {"mag_evals_per_s_fwd_vjp": L_ * T / dt, "ms": dt * 1e3}
-    # roofline: the kernel solves the epochs' lens polynomials with the same solver and
-    # initial estimates as caus_roots, so the root-iterations of each form are counted
-    # by solving the same polynomials with the counting entry point
+    # roofline: each lane of the kernel takes epl consecutive epochs of one light curve,
+    # the first solved from the initial estimates of caus_roots, each later one started
+    # from the previous epoch's roots.  The root-iterations of each form are counted by
+    # replaying that schedule through the counting entry point: round r solves the
+    # epochs with index = r (mod epl), started from round r-1's roots.
     from caustics_amd.ehrlich_aberth_primitive import solve_with_stats
     tt = t.cpu().numpy()[None, :]
     pr = {k: np.asarray(prm[k], float)[:, None] for k in ("t0", "u0", "tE", "alpha", "s", "q")}
@@ -414,16 +422,28 @@ def bench_lightcurves(steps, warmup, host_check=True, cpu_s=0.0):
     lf, x_cm = W.lens_frame({"s": pr["s"], "q": pr["q"]}, 2)
     c = W.coeffs_product((w_user + x_cm).reshape(-1), np.broadcast_to(lf["a"], w_user.shape).reshape(-1),
                          np.broadcast_to(lf["e1"], w_user.shape).reshape(-1), None, None, 2)
+    c = c.reshape(L_, T, -1)
+    epl = min(8, max(1, -(-(T * L_) // 131072)))   # lc_epochs_per_lane of the kernel
     for name in ("fp64", "cf32"):
-        _, _, st = solve_with_stats(c, mode=name, init="complex")
-        sw = st.sum(0).cpu().numpy().astype(np.float64)
+        sw = np.zeros(4)
+        z = None
+        for r in range(epl):
+            cr = torch.as_tensor(c[:, r::epl].reshape(-1, c.shape[-1]), device="cuda")
+            if z is not None:
+                z = z.reshape(L_, -1, z.shape[-1])[:, :cr.shape[0] // L_].reshape(-1, z.shape[-1])
+            z, _, st = solve_with_stats(cr, mode=name, init="complex", roots_init=z)
+            sw += st.sum(0).cpu().numpy().astype(np.float64)
         work = [sw[1], sw[3], sw[2], float(L_ * T), 0.0]
         f64, f32 = contour_flops(work, 2, False)
         f64 += pointsource_vjp_flops(L_ * T, 2, 6)
         out[name]["roofline"] = _roofline_entry((f64, f32), out[name]["ms"] * 1e-3)
-        out[name]["roofline"]["rule"] = ("root-iterations x 160 + per epoch 1e3 (coefficients) "
-                                         "+ 5 x 65 (lens equation, det J) + VJP 5 roots x 6 "
-                                         "directions x 50")
+        out[name]["roofline"]["work"] = {"root_iterations_fp64": int(sw[1]),
+                                         "root_iterations_fp32": int(sw[3]),
+                                         "root_iterations_comp": int(sw[2]), "epochs": L_ * T}
+        out[name]["roofline"]["rule"] = (
+            f"root-iterations x {flops_per_root_iter(5)} (FP32 ones against the FP32 peak) + "
+            f"per epoch {COEFF_FLOPS[2]} (coefficients) + 5 x {IMAGE_FLOPS[2]} (image terms) "
+            "+ VJP 5 roots x 6 directions x 50")
     m64, g64 = mags["fp64"]
     m32, g32 = mags["cf32"]
     wtraj = W.trajectory(tt[0], prm["t0"], prm["u0"], prm["tE"], prm["alpha"])
@@ -451,15 +471,15 @@ def contour_flops(work, nlenses, ld, npts_ld=100, vjp=False):
     """Algorithmic flops of a contour run from its work counters (FP64 root-iterations,
     FP32 root-iterations, compensated root-iterations, limb-point solves, contour
     points), by the counting rule of the root solver plus, per limb-point solve, the
-    coefficient build (about 10^3 flops binary, 10^4 triple) and n x (lens equation 25 +
-    det J 40), and per limb-darkened contour point 2 npts_ld Gauss-Legendre nodes
+    coefficient build (COEFF_FLOPS) and n x the image terms (IMAGE_FLOPS: lens equation
+    and det J), and per limb-darkened contour point 2 npts_ld Gauss-Legendre nodes
     (NODE_FLOPS_FWD, or NODE_FLOPS_VJP with the reverse mode, which also adds the image
     adjoint of every contour point).  The compensated polish that ends every limb solve
     is this implementation's addition and is not counted.  Returns (FP64, FP32 flops)."""
     n = nlenses * nlenses + 1
     it64, it32, itc, solves, points = (float(x) for x in work)
     f64 = it64 * flops_per_root_iter(n) + itc * flops_comp_root_iter(n)
-    f64 += solves * ((1e3 if nlenses == 2 else 1e4) + n * 65)
+    f64 += solves * (COEFF_FLOPS[nlenses] + n * IMAGE_FLOPS[nlenses])
     node = (NODE_FLOPS_VJP if vjp else NODE_FLOPS_FWD)[nlenses]
     f64 += points * (2 * npts_ld * node if ld else 4)
     if vjp:
@@ -480,7 +500,8 @@ def _roofline_entry(flops, dt):
     f64, f32 = flops
     busy = (f64 / (FP64_PEAK_TFLOPS * 1e12) + f32 / (FP32_PEAK_TFLOPS * 1e12)) / dt
     return {"bound": "valu", "achieved_fp64_tflops": f64 / dt / 1e12,
-            "achieved_fp32_tflops": f32 / dt / 1e12, "frac": busy}
+            "achieved_fp32_tflops": f32 / dt / 1e12, "frac": busy,
+            "fp64_flops": f64, "fp32_flops": f32, "seconds": dt}
 
 
 # ---- CPU baselines: the oracle (NumPy restatement + the C root oracle) on the host ----

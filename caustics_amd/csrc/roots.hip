@@ -4,8 +4,9 @@
 // cpu_ops.cc:15-81) and gpu_ehrlich_aberth (lib/ehrlich_aberth/kernels.cc.cu:52-104):
 // coefficients (N, deg+1) complex128 ascending, optional initial roots (N, deg),
 // output roots (N, deg).  One polynomial per lane, 256 lanes per workgroup, no
-// allocation, no device synchronisation.  The only scratch memory is init_est's
-// log-magnitude table (read at runtime hull vertices), (deg+1) doubles per lane.
+// allocation, no device synchronisation.  Scratch memory: the reference-order (RefCompat)
+// initial estimates read their log-magnitude table at runtime hull vertices, (deg+1)
+// doubles per lane; the Fast form's estimates and sweeps stay in registers.
 #include <atomic>
 
 #include "caustics_hip.h"

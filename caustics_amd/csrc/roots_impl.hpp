@@ -186,13 +186,14 @@ struct CompactEA {
 // polynomial per lane; lanes refill (stride L) only if the grid had to be capped.  With
 // init null every lane computes its own initial estimates (init_est.h:57-102, the same
 // function and inputs as k_roots_init, so the same doubles): all lanes start together,
-// so this adds no divergence, and no estimates pass through HBM.  Inlined, the estimate
-// code raises the kernel's register allocation (degree 10: 186 VGPRs, 2 waves per
-// SIMD, against 166 and 3 without it), yet the fused launch measured 4% (degree 10) and
-// 7% (degree 5) faster than the init kernel plus this one; a called, non-inlined
-// init_est was more than twice as slow (its stack frame goes through scratch).
+// so this adds no divergence, and no estimates pass through HBM.  The estimates use the
+// compile-time-indexed hull (init_est_static), so nothing goes through scratch; the
+// kernel is held to 3 waves per SIMD (degree 10: 167 VGPRs, 3 dwords spilled, against
+// 183 VGPRs and 2 waves unconstrained), measured 3% faster at degree 10 with results
+// bit for bit the same.  A called, non-inlined init_est was more than twice as slow
+// (its stack frame goes through scratch).
 template <int DEG>
-__global__ __launch_bounds__(kRootsBlock) void k_roots_compact(
+__global__ __launch_bounds__(kRootsBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_roots_compact(
     const cd* __restrict__ coeffs, const cd* init, cd* roots, int32_t* __restrict__ stats,
     uint8_t* __restrict__ status, int64_t N, int itmax) {
     __shared__ double2 zs[DEG * kRootsBlock];

@@ -8,8 +8,8 @@
 //                              lib/ehrlich_aberth/horner.h:281-385
 // Layout: coefficients ascending (p[k] multiplies z^k), held in VGPRs; every
 // loop over the degree is unrolled at compile time, so the iteration's arrays stay in
-// registers (init_est's log-magnitude table, read at runtime hull vertices, is the
-// one array the compiler places in scratch).
+// registers (the RefCompat form's init_est reads its log-magnitude table at runtime hull
+// vertices, the one array the compiler places in scratch).
 //
 // Two iteration forms:
 //   RefCompat  the reference's branch structure: |z|>1 uses the reversed polynomial
@@ -49,11 +49,73 @@ __device__ __forceinline__ double sel(const double (&a)[N], int idx) {
 // from x = DEG down to 0 exactly as convex_hull (init_est.h:37-54) does, so the top of
 // the stack is the lowest set bit.  Root slot m lies on the hull edge [xlo, xhi)
 // containing m, which is where the reference's running offset k places it.
+// Fast form: the same hull found with compile-time indices only, so that the table
+// stays in registers.  Point i (0 < i < DEG) is a vertex of the upper hull exactly when
+// a line through it passes strictly above every other point, i.e. when every slope
+// from a point on its left exceeds every slope to a point on its right; the end points
+// are always vertices.  (The monotone chain pops collinear points, so equality is no
+// vertex in either test; the two tests can differ only by the rounding of nearly
+// collinear triples, which moves an initial estimate, not a root.)  Root slot m then
+// lies on the edge [xlo(m), xhi(m)) of the nearest vertices at or below and above m.
+template <int DEG>
+__device__ __forceinline__ void init_est_static(const double (&y)[DEG + 1], cd (&z)[DEG]) {
+    bool vx[DEG + 1];
+    vx[0] = vx[DEG] = true;
+#pragma unroll
+    for (int i = 1; i < DEG; ++i) {
+        double lo = (y[i] - y[0]) * (1.0 / i), hi = (y[DEG] - y[i]) * (1.0 / (DEG - i));
+#pragma unroll
+        for (int a = 1; a < i; ++a) lo = fmin(lo, (y[i] - y[a]) * (1.0 / (i - a)));
+#pragma unroll
+        for (int b = i + 1; b < DEG; ++b) hi = fmax(hi, (y[b] - y[i]) * (1.0 / (b - i)));
+        vx[i] = hi < lo;
+    }
+    // nearest vertex above each slot, and the number of vertices above it
+    int xhi[DEG], nabove[DEG];
+    double yhi[DEG];
+    {
+        int x = DEG, na = 1;   // DEG is a vertex
+        double yy = y[DEG];
+#pragma unroll
+        for (int m = DEG - 1; m >= 0; --m) {
+            xhi[m] = x;
+            yhi[m] = yy;
+            nabove[m] = na;
+            if (vx[m]) {
+                ++na;
+                x = m;
+                yy = y[m];
+            }
+        }
+    }
+    const double sigma_pi = 0.7 * 0.31830988618379067154;
+    int xlo = 0;
+    double ylo = y[0];
+#pragma unroll
+    for (int m = 0; m < DEG; ++m) {
+        if (vx[m]) {
+            xlo = m;
+            ylo = y[m];
+        }
+        const int nz = xhi[m] - xlo;
+        const double r = exp((ylo - yhi[m]) / nz);
+        // e = vertices strictly above xhi: nabove[m] counts those above m, xhi included
+        const int e = nabove[m] - 1;
+        double s, c;
+        sincospi((2.0 / nz) * (m - xlo) + (2.0 / DEG) * e + sigma_pi, &s, &c);
+        z[m] = cd{r * c, r * s};
+    }
+}
+
 template <int DEG, bool REALAXIS>
 __device__ __forceinline__ void init_est(const double (&alpha)[DEG + 1], cd (&z)[DEG]) {
     double y[DEG + 1];
 #pragma unroll
     for (int i = 0; i <= DEG; ++i) y[i] = alpha[i] > 0 ? log(alpha[i]) : -1e30;
+    if constexpr (!REALAXIS) {
+        init_est_static<DEG>(y, z);
+        return;
+    }
     unsigned st = 0;
 #pragma unroll
     for (int i = DEG; i >= 0; --i) {

@@ -87,6 +87,29 @@ def test_all_degrees_known_roots(gpu, deg):
         assert ok.all()
 
 
+@pytest.mark.parametrize("deg", [2, 3, 5, 7, 10, 12])
+def test_initial_estimates_match_oracle_hull(gpu, deg):
+    """itmax = 0 returns the initial estimates.  Both forms place them on the upper convex
+    hull of (i, log|p_i|) (init_est.h:57-102): the reference form by the monotone chain,
+    the Fast form by its compile-time-indexed vertex test.  Against the oracle's
+    restatement (ora_init_est, init_mode 0 real-axis, 1 complex): a wrong hull vertex or
+    slot assignment moves an estimate by O(1), rounding by ~1e-15.  Coefficient
+    magnitudes span 1e-13..1e13 with exact zeros, so hulls have many edges."""
+    g = np.random.default_rng(100 + deg)
+    n = 4000
+    mag = np.exp(g.normal(0, 10, (n, deg + 1)))
+    mag[g.random((n, deg + 1)) < 0.1] = 0.0
+    mag[:, 0] = np.maximum(mag[:, 0], 1e-3)
+    mag[:, deg] = np.maximum(mag[:, deg], 1e-3)
+    c = mag * np.exp(2j * np.pi * g.random((n, deg + 1)))
+    for init, mode in (("reference", 0), ("complex", 1)):
+        z, _, _ = solve_with_stats(c, itmax=0, mode="fp64", init=init)
+        zo = ea.roots_asc(c, itmax=0, init_mode=mode)
+        rel = np.abs(z.cpu().numpy() - zo) / np.abs(zo)
+        print(f"deg {deg} {init}: max rel diff {rel.max():.2e}")
+        assert rel.max() <= 1e-12
+
+
 def _known_root_polys(n, deg, seed, scale=1.0):
     g = np.random.default_rng(seed)
     zs = scale * (g.normal(size=(n, deg)) + 1j * g.normal(size=(n, deg)))
