@@ -434,14 +434,15 @@ def bench_lightcurves(steps, warmup, host_check=True, cpu_s=0.0):
             z, _, st = solve_with_stats(cr, mode=name, init="complex", roots_init=z)
             sw += st.sum(0).cpu().numpy().astype(np.float64)
         work = [sw[1], sw[3], sw[2], float(L_ * T), 0.0]
-        f64, f32 = contour_flops(work, 2, False)
+        f64, f32 = contour_flops(work, 2, False, cf32=name == "cf32")
         f64 += pointsource_vjp_flops(L_ * T, 2, 6)
         out[name]["roofline"] = _roofline_entry((f64, f32), out[name]["ms"] * 1e-3)
         out[name]["roofline"]["work"] = {"root_iterations_fp64": int(sw[1]),
                                          "root_iterations_fp32": int(sw[3]),
                                          "root_iterations_comp": int(sw[2]), "epochs": L_ * T}
         out[name]["roofline"]["rule"] = (
-            f"root-iterations x {flops_per_root_iter(5)} (FP32 ones against the FP32 peak) + "
+            f"root-iterations x {flops_per_root_iter(5)} (FP32 ones against the FP32 peak; "
+            f"CF32 compensated FP32 sweeps x {flops_cf32_comp_root_iter(5)}) + "
             f"per epoch {COEFF_FLOPS[2]} (coefficients) + 5 x {IMAGE_FLOPS[2]} (image terms) "
             "+ VJP 5 roots x 6 directions x 50")
     m64, g64 = mags["fp64"]
@@ -467,7 +468,7 @@ def bench_lightcurves(steps, warmup, host_check=True, cpu_s=0.0):
 FP32_PEAK_TFLOPS = 157.3
 
 
-def contour_flops(work, nlenses, ld, npts_ld=100, vjp=False):
+def contour_flops(work, nlenses, ld, npts_ld=100, vjp=False, cf32=False):
     """Algorithmic flops of a contour run from its work counters (FP64 root-iterations,
     FP32 root-iterations, compensated root-iterations, limb-point solves, contour
     points), by the counting rule of the root solver plus, per limb-point solve, the
@@ -475,16 +476,17 @@ def contour_flops(work, nlenses, ld, npts_ld=100, vjp=False):
     and det J), and per limb-darkened contour point 2 npts_ld Gauss-Legendre nodes
     (NODE_FLOPS_FWD, or NODE_FLOPS_VJP with the reverse mode, which also adds the image
     adjoint of every contour point).  The compensated polish that ends every limb solve
-    is this implementation's addition and is not counted.  Returns (FP64, FP32 flops)."""
+    is this implementation's addition and is not counted.  With cf32 the compensated
+    root-iterations are the CF32 form's FP32 sweep.  Returns (FP64, FP32 flops)."""
     n = nlenses * nlenses + 1
     it64, it32, itc, solves, points = (float(x) for x in work)
-    f64 = it64 * flops_per_root_iter(n) + itc * flops_comp_root_iter(n)
+    f64 = it64 * flops_per_root_iter(n) + (0.0 if cf32 else itc * flops_comp_root_iter(n))
     f64 += solves * (COEFF_FLOPS[nlenses] + n * IMAGE_FLOPS[nlenses])
     node = (NODE_FLOPS_VJP if vjp else NODE_FLOPS_FWD)[nlenses]
     f64 += points * (2 * npts_ld * node if ld else 4)
     if vjp:
         f64 += points * IMAGE_ADJ_FLOPS[nlenses]
-    f32 = it32 * flops_per_root_iter(n)
+    f32 = it32 * flops_per_root_iter(n) + (itc * flops_cf32_comp_root_iter(n) if cf32 else 0.0)
     return f64, f32
 
 
@@ -542,6 +544,14 @@ def _oracle_extended_pair(kw):
     """(oracle as the reference computes it, oracle with the canonical root polish)"""
     from oracle import extended as OE
     return OE.mag_extended_source(**kw), OE.mag_extended_source(polish=True, **kw)
+
+
+def _oracle_truth(kw):
+    """source-plane quadrature of the same magnification at two resolutions"""
+    from oracle import extended as OE
+    k = {n: kw[n] for n in ("s", "q", "q3", "r3", "psi") if n in kw}
+    return [OE.mag_disk_quadrature(kw["w0"], kw["rho"], kw["nlenses"], u1=kw["u1"], nr=nr, **k)
+            for nr in (100, 200)]
 
 
 def _oracle_point_source(args):
@@ -627,7 +637,8 @@ def bench_extended_binary(steps, warmup, n_near=2048, n_far=2048, cpu_s=0.0):
                      "work": {"root_iterations_fp64": int(work[0]),
                               "root_iterations_fp32": int(work[1]),
                               "limb_solves": int(work[3]), "contour_points": int(work[4])},
-                     "roofline": _roofline_entry(contour_flops(work, 2, True, vjp=True), dt),
+                     "roofline": _roofline_entry(contour_flops(work, 2, True, vjp=True,
+                                                               cf32=form == "cf32"), dt),
                      "grad_finite": bool(torch.isfinite(go).all().item())}
     m64, m32 = res["fp64"][0], res["cf32"][0]
     e = ((m32 - m64).abs() / m64.abs()).cpu().numpy()
@@ -694,7 +705,8 @@ def bench_triple(steps, warmup, n_sets=1024, n_epochs=1500, npts=256, budget_s=4
         work = work.cpu().numpy() * (S / (8 * n_epochs))
         out[form] = {"mag_evals_per_s_fwd_vjp": S / dt, "ms": dt * 1e3,
                      "unconverged": int((res[form][1] == 0).sum().item()),
-                     "roofline": _roofline_entry(contour_flops(work, 3, True, vjp=True), dt),
+                     "roofline": _roofline_entry(contour_flops(work, 3, True, vjp=True,
+                                                               cf32=form == "cf32"), dt),
                      "roofline_note": "work counters from the first 8 parameter sets, scaled"}
     m64, m32 = res["fp64"][0], res["cf32"][0]
     e = ((m32 - m64).abs() / m64.abs()).cpu().numpy()
@@ -725,10 +737,15 @@ def bench_triple(steps, warmup, n_sets=1024, n_epochs=1500, npts=256, budget_s=4
         with mp.get_context("spawn").Pool(host_cores()[0], initializer=_pool_init,
                                           initargs=(ROOT,)) as pool:
             pairs = np.array(pool.map(_oracle_extended_pair, samp, chunksize=1))
+            truth = np.array(pool.map(_oracle_truth, samp, chunksize=1))
         gi = np.array([kk * n_epochs + e for kk, e in rows])
         g64 = m64.cpu().numpy()[gi]
         e_p = np.abs(g64 - pairs[:, 0]) / pairs[:, 0]
         e_c = np.abs(g64 - pairs[:, 1]) / pairs[:, 1]
+        tq = truth[:, 1]
+        conv = np.abs(truth[:, 0] - tq) <= 1e-7 * tq
+        eg_t = np.abs(g64 - tq) / tq
+        eo_t = np.abs(pairs[:, 0] - tq) / tq
         out["fp64"]["vs_oracle"] = {
             "epochs": len(rows), "sample": "epochs 300, 744, 749, 753, 1200 of the first "
                                            f"{min(k, 32)} light curves (crossing near 749.5)",
@@ -737,7 +754,19 @@ def bench_triple(steps, warmup, n_sets=1024, n_epochs=1500, npts=256, budget_s=4
             "max_rel_err": float(e_p.max()), "p99_rel_err": float(np.quantile(e_p, 0.99)),
             "polished_oracle": {"frac_le_1e-10": float(np.mean(e_c <= 1e-10)),
                                 "frac_le_1e-8": float(np.mean(e_c <= 1e-8)),
-                                "max_rel_err": float(e_c.max())}}
+                                "max_rel_err": float(e_c.max())},
+            # both against the source-plane quadrature of the same disk (oracle/extended.py
+            # mag_disk_quadrature, 200 nodes; "converged" where 100 nodes agree to 1e-7)
+            "vs_truth": {
+                "converged_epochs": int(conv.sum()),
+                "gpu_median_rel_err_converged": float(np.median(eg_t[conv])) if conv.any() else None,
+                "oracle_median_rel_err_converged": float(np.median(eo_t[conv])) if conv.any() else None,
+                "gpu_max_rel_err_converged": float(eg_t[conv].max()) if conv.any() else None,
+                "oracle_max_rel_err_converged": float(eo_t[conv].max()) if conv.any() else None,
+                "frac_gpu_oracle_gap_within_oracle_error": float(
+                    np.mean(np.abs(g64 - pairs[:, 0]) <= np.abs(pairs[:, 0] - tq))),
+                "where_gpu_differs_gpu_closer": int(((e_p > 1e-10) & (eg_t < eo_t)).sum()),
+                "where_gpu_differs_oracle_closer": int(((e_p > 1e-10) & (eo_t < eg_t)).sum())}}
     return out
 
 

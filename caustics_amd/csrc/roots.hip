@@ -87,6 +87,22 @@ extern "C" int caus_gpu_ehrlich_aberth(void* stream, void** buffers, const char*
                       d.compensated ? CAUS_MODE_FP64_COMP : CAUS_MODE_FP64, CAUS_INIT_REFERENCE);
 }
 
+// The reference target's exact signature, void return (kernels.h:19-20): an XLA
+// custom-call target cannot return a value, so the status of the last call on this
+// host thread is kept for caus_gpu_ehrlich_aberth_status.
+static thread_local int g_gpu_ea_status = CAUS_OK;
+
+extern "C" void caus_gpu_ehrlich_aberth_void(void* stream, void** buffers, const char* opaque,
+                                             size_t opaque_len) {
+    g_gpu_ea_status = caus_gpu_ehrlich_aberth(stream, buffers, opaque, opaque_len);
+}
+
+extern "C" int caus_gpu_ehrlich_aberth_status(void) {
+    const int s = g_gpu_ea_status;
+    g_gpu_ea_status = CAUS_OK;
+    return s;
+}
+
 namespace caus {
 #define CAUS_DECL_VJP(D) \
     extern template int roots_vjp_for_degree<D>(hipStream_t, const cd*, const cd*, const cd*, cd*, int64_t);
@@ -125,6 +141,20 @@ extern "C" int caus_roots_vjp(void* stream, const double* coeffs, const double* 
         default:
             return CAUS_ERR_ARG;
     }
+}
+
+// caus_roots_vjp as an XLA custom-call target beside caus_gpu_ehrlich_aberth_void:
+// buffers = {coeffs, roots, g_roots, g_coeffs}, opaque = the descriptor (size, deg used).
+extern "C" void caus_gpu_ehrlich_aberth_vjp_void(void* stream, void** buffers,
+                                                 const char* opaque, size_t opaque_len) {
+    if (opaque_len != sizeof(CausEhrlichAberthDescriptor) || !buffers) {
+        g_gpu_ea_status = CAUS_ERR_ARG;
+        return;
+    }
+    CausEhrlichAberthDescriptor d;
+    __builtin_memcpy(&d, opaque, sizeof(d));
+    g_gpu_ea_status = caus_roots_vjp(stream, (const double*)buffers[0], (const double*)buffers[1],
+                                     (const double*)buffers[2], (double*)buffers[3], d.size, d.deg);
 }
 
 extern "C" int caus_abi_version(void) { return CAUS_ABI_VERSION; }

@@ -72,6 +72,14 @@ int caus_roots(void* stream, const double* coeffs, const double* roots_init, dou
  * CausEhrlichAberthDescriptor.  Uses CAUS_INIT_REFERENCE. */
 int caus_gpu_ehrlich_aberth(void* stream, void** buffers, const char* opaque, size_t opaque_len);
 
+/* The same with the reference target's exact signature and void return
+ * (lib/ehrlich_aberth/kernels.h:19-20, registered by gpu_ops.cc:14-26): what an XLA
+ * custom-call registration binds.  The status code of the last call on the calling host
+ * thread is read (and reset to CAUS_OK) by caus_gpu_ehrlich_aberth_status. */
+void caus_gpu_ehrlich_aberth_void(void* stream, void** buffers, const char* opaque,
+                                  size_t opaque_len);
+int caus_gpu_ehrlich_aberth_status(void);
+
 /* Coefficient gradient of the root map: the adjoint of the implicit-function JVP
  * _ehrlich_aberth_jvp (src/caustics/ehrlich_aberth_primitive.py:254-324),
  * dz_j/dp_k = -z_j^k / p'(z_j).  With cotangents g_roots (N, deg) it writes
@@ -80,6 +88,14 @@ int caus_gpu_ehrlich_aberth(void* stream, void** buffers, const char* opaque, si
  * returned by caus_roots for the same coefficients. */
 int caus_roots_vjp(void* stream, const double* coeffs, const double* roots, const double* g_roots,
                    double* g_coeffs, int64_t N, int32_t deg);
+
+/* caus_roots_vjp as a custom-call target of the same shape as
+ * caus_gpu_ehrlich_aberth_void, for the transpose of the primitive's JVP
+ * (ehrlich_aberth_primitive.py:254-324) on the JAX side: buffers = {coeffs, roots,
+ * g_roots, g_coeffs}, opaque = CausEhrlichAberthDescriptor bytes (size and deg are
+ * used).  Status as caus_gpu_ehrlich_aberth_status. */
+void caus_gpu_ehrlich_aberth_vjp_void(void* stream, void** buffers, const char* opaque,
+                                      size_t opaque_len);
 
 /* Point-source magnification, fused (reference: mag_point_source,
  * src/caustics/point_source.py:1762-1830, with _images_point_source :1652-1709,

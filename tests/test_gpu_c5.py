@@ -15,7 +15,10 @@ refinement round picks, can turn on rounding.  Every epoch is therefore checked 
 root polish (oracle/ea_oracle.c: ora_polish), and (c) copies of (a) whose polynomial
 coefficients are perturbed by 1e-16 (the spread of results the FP64 route itself gives);
 an epoch passes when the GPU is within 1e-10 of (a) or (b), or within 10x the spread of
-(c) around (a).  The bulk must agree to 1e-10 outright."""
+(c) around (a).  The bulk must agree to 1e-10 outright.  Beyond these, the GPU may differ
+from (a) by no more than (a) differs from the magnification itself (the source-plane
+quadrature oracle/extended.py: mag_disk_quadrature): where the reference's route is that
+far off, its digits carry no information the product could reproduce."""
 import numpy as np
 import pytest
 import torch
@@ -66,14 +69,23 @@ def test_triple_ld_epochs_match_oracle(gpu):
             spread = max(abs(r - plain[i]) for r in runs)
             ok_spread[i] = abs(m[i] - plain[i]) <= max(1e-10 * plain[i], 10 * spread)
     ok = ok_plain | ok_pol | ok_spread
+    truth = np.full(n, np.nan)
+    ok_truth = np.zeros(n, bool)
+    for i in np.nonzero(~ok)[0]:
+        truth[i] = E.mag_disk_quadrature(w[i], rho[i], 3, u1=u1[i], nr=200,
+                                         **{k: prm[k][i] for k in NAMES})
+        ok_truth[i] = abs(m[i] - plain[i]) <= abs(plain[i] - truth[i])
     print(f"C5 epochs: {n}; within 1e-10 of the oracle {ok_plain.mean():.3f}, of the polished "
           f"oracle {(ok_plain | ok_pol).mean():.3f}, within the FP64 route's spread "
-          f"{ok.mean():.3f}; max rel err vs oracle {e_plain.max():.2e}")
+          f"{ok.mean():.3f}, or closer to the oracle than the oracle is to the quadrature "
+          f"{(ok | ok_truth).mean():.3f}; max rel err vs oracle {e_plain.max():.2e}")
     for i in np.nonzero(~ok)[0]:
-        print(f"  epoch {i}: gpu {m[i]:.12g} oracle {plain[i]:.12g} polished {polished[i]:.12g}")
-    # observed on these 72 epochs: 0.82, 0.90 and a largest deviation of 2.3e-5
+        print(f"  epoch {i}: gpu {m[i]:.12g} oracle {plain[i]:.12g} polished {polished[i]:.12g} "
+              f"quadrature {truth[i]:.12g}")
+    # observed on these 72 epochs: 0.76, 0.82, 0.90 and 1.0, largest deviation 2.3e-5
     assert (ok_plain | ok_pol).mean() >= 0.75
     assert ok.mean() >= 0.85
+    assert (ok | ok_truth).all()
     assert e_plain.max() <= 2e-4
 
 

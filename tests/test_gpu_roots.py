@@ -231,6 +231,15 @@ def test_descriptor_entry_point(gpu):
     assert rc == 0
     z = ca.ehrlich_aberth(c).cpu().numpy().reshape(1000, 5)
     np.testing.assert_array_equal(out.cpu().numpy(), z)
+    # the void-returning form of the reference target's signature
+    L = _lib.lib()
+    out.zero_()
+    L.caus_gpu_ehrlich_aberth_void(_lib.stream_ptr(), bufs, raw, len(raw))
+    assert L.caus_gpu_ehrlich_aberth_status() == 0
+    np.testing.assert_array_equal(out.cpu().numpy(), z)
+    L.caus_gpu_ehrlich_aberth_void(_lib.stream_ptr(), bufs, raw, len(raw) - 1)   # bad size
+    assert L.caus_gpu_ehrlich_aberth_status() == _lib.CAUS_ERR_ARG
+    assert L.caus_gpu_ehrlich_aberth_status() == 0   # read resets
 
 
 def test_roots_vjp_matches_implicit_derivative_and_fd(gpu):
@@ -253,6 +262,17 @@ def test_roots_vjp_matches_implicit_derivative_and_fd(gpu):
     gre = np.real(v[..., None] * J).sum(1)
     gim = np.real(v[..., None] * 1j * J).sum(1)
     np.testing.assert_allclose(grad, gre + 1j * gim, rtol=1e-9, atol=1e-12)
+    # the custom-call-shaped target: buffers {coeffs, roots, g_roots, g_coeffs}; the
+    # cotangent of z for L = Re(v z) is conj(v) in torch's convention
+    zt = z.detach().contiguous()
+    gz = torch.as_tensor(np.conj(v), device="cuda")
+    gc = torch.zeros_like(ct)
+    bufs = (ctypes.c_void_p * 4)(ct.data_ptr(), zt.data_ptr(), gz.data_ptr(), gc.data_ptr())
+    raw = bytes((ctypes.c_int * 3)(64, 5, 2000)) + bytes(4)
+    L = _lib.lib()
+    L.caus_gpu_ehrlich_aberth_vjp_void(_lib.stream_ptr(), bufs, raw, len(raw))
+    assert L.caus_gpu_ehrlich_aberth_status() == 0
+    np.testing.assert_array_equal(gc.cpu().numpy(), grad)
     # central finite differences through the oracle solver (h = 1e-6)
     h = 1e-6
     for kk in (0, 3, 5):

@@ -89,6 +89,23 @@ def test_single_lens_ld_closed_form_and_u1_zero(rho):
     np.testing.assert_allclose(m0, mu, rtol=1e-3)
 
 
+@pytest.mark.parametrize("rho", [0.1, 0.01])
+def test_disk_quadrature_single_lens_closed_forms(rho):
+    """The truth route (oracle/extended.py: mag_disk_quadrature) against the single-lens
+    disk integrals above: spectral accuracy where the disk is clear of the point caustic
+    (observed 2e-15..6e-15), slow convergence where it holds it (observed 6.7e-3 at 40
+    nodes, 4.3e-3 at 80), which is why callers compare two resolutions there."""
+    for u in (1.5 * rho, 3.0 * rho):
+        for u1 in (0.0, 0.7):
+            ex = single_ld(u, rho, u1) if u1 else single_uniform(u, rho)
+            m = E.mag_disk_quadrature(u, rho, nlenses=1, u1=u1, nr=40)
+            assert abs(m / ex - 1) < 1e-13
+    ex = single_uniform(0.5 * rho, rho)
+    e40 = abs(E.mag_disk_quadrature(0.5 * rho, rho, nlenses=1, nr=40) / ex - 1)
+    e80 = abs(E.mag_disk_quadrature(0.5 * rho, rho, nlenses=1, nr=80) / ex - 1)
+    assert e80 < 0.7 * e40 < 1e-2
+
+
 def source_plane_quadrature(w0, rho, nr=500, nphi=1000, **params):
     """Midpoint rule in polar coordinates over the source disk of the point-source
     magnification (an independent route to the same integral)."""
