@@ -659,11 +659,15 @@ def bench_extended_binary(steps, warmup, n_near=2048, n_far=2048, cpu_s=0.0):
     return out
 
 
-def bench_triple(steps, warmup, n_sets=1024, n_epochs=1500, npts=256, budget_s=40.0, cpu_s=0.0):
+def bench_triple(steps, warmup, n_sets=1024, n_epochs=1500, budget_s=40.0, cpu_s=0.0):
     """Triple-lens caustic-crossing light curves (u=0.4, rho log-uniform [1e-3,3e-2]),
-    full limb-darkened contour at every epoch, fused fwd+VJP, both solver forms.  The
-    timed batch is the full n_sets x n_epochs when one step fits budget_s, otherwise the
-    largest multiple of 8 parameter sets that does (stated in the result)."""
+    full limb-darkened contour at every epoch with uneven limb resolutions (256 far from
+    the crossing up to 4096 within one source radius, W.crossing_npts), fused fwd+VJP,
+    both solver forms.  The whole batch (ragged ordering, chunks on two streams, every
+    kernel) is captured once in a HIP graph and the timed step is one replay, checked bit
+    for bit against the eager call.  The timed batch is the full n_sets x n_epochs when one
+    step fits budget_s, otherwise the largest multiple of 8 parameter sets that does
+    (stated in the result)."""
     import torch
 
     from caustics_amd import _lib
@@ -671,68 +675,95 @@ def bench_triple(steps, warmup, n_sets=1024, n_epochs=1500, npts=256, budget_s=4
     from caustics_amd.extended_source import _run
 
     b = W.triple_caustic_lightcurves(n_sets, n_epochs)
+    npts_all = W.crossing_npts(b)
+    NMAX = int(npts_all.max())
 
     def batch(k):
         w = torch.as_tensor(b["w"][:k].reshape(-1), device="cuda")
         rep = lambda x: torch.as_tensor(np.repeat(np.asarray(x)[:k], n_epochs), device="cuda")
         prm = torch.stack([rep(b[c]) for c in ("s", "q", "q3", "r3", "psi")], -1).contiguous()
-        return w, rep(b["rho"]), rep(b["u1"]), prm
+        return w, rep(b["rho"]), rep(b["u1"]), prm, npts_all[:k].reshape(-1).astype(np.int64)
 
     # size the timed batch from a probe of 8 parameter sets
-    w, rho, u1, prm = batch(8)
+    w, rho, u1, prm, nh = batch(8)
     gm = torch.ones(w.numel(), dtype=torch.float64, device="cuda")
-    _run(3, w, rho, prm, None, npts, True, u1, 100, 2500, _lib.MODE_FP64, g=gm)
+    _run(3, w, rho, prm, nh, NMAX, True, u1, 100, 2500, _lib.MODE_FP64, g=gm)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    _run(3, w, rho, prm, None, npts, True, u1, 100, 2500, _lib.MODE_FP64, g=gm)
+    _run(3, w, rho, prm, nh, NMAX, True, u1, 100, 2500, _lib.MODE_FP64, g=gm)
     torch.cuda.synchronize()
     per_set = (time.perf_counter() - t0) / 8
     k = int(min(n_sets, max(8, budget_s / 2.5 / max(per_set, 1e-9))) // 8 * 8)
-    w, rho, u1, prm = batch(k)
+    w, rho, u1, prm, nh = batch(k)
     S = w.numel()
     gm = torch.ones(S, dtype=torch.float64, device="cuda")
+    hist = {int(v): int(c) for v, c in zip(*np.unique(nh, return_counts=True))}
     out = {"workload": f"{k} of {n_sets} triple-lens parameter sets x {n_epochs} epochs "
-                       f"(caustic crossing), limb-darkened u=0.4, npts_limb={npts}, full "
-                       "contour fwd+VJP at every epoch", "evals": S}
+                       "(caustic crossing), limb-darkened u=0.4, npts_limb 256..4096 by distance "
+                       "to the crossing, full contour fwd+VJP at every epoch, one HIP graph "
+                       "replay", "evals": S, "npts_limb_histogram": hist,
+           "limb_points": int(nh.sum())}
     res = {}
     for form, mode in (("fp64", _lib.MODE_FP64), ("cf32", _lib.MODE_CF32)):
-        def step():
-            res[form] = _run(3, w, rho, prm, None, npts, True, u1, 100, 2500, mode, g=gm)
-        dt = _timed(step, 1, 0, f"C5 triple contours {form} ({S} contours)")
+        def call():
+            res[form] = _run(3, w, rho, prm, nh, NMAX, True, u1, 100, 2500, mode, g=gm)
+        t0 = time.perf_counter()
+        call()
+        torch.cuda.synchronize()
+        eager_s = time.perf_counter() - t0
+        eager = [x.clone() for x in (res[form][0], res[form][2])]
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            call()
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            call()
+        captured = res[form]
+        dt = _timed(graph.replay, 1, 1, f"C5 triple contours {form} graph replay ({S} contours)")
+        same = bool(torch.equal(captured[0], eager[0]) and torch.equal(captured[2], eager[1]))
         work = torch.zeros(5, dtype=torch.int64, device="cuda")
-        _run(3, w[:8 * n_epochs], rho[:8 * n_epochs], prm[:8 * n_epochs], None, npts, True,
-             u1[:8 * n_epochs], 100, 2500, mode, g=gm[:8 * n_epochs], work=work)
-        work = work.cpu().numpy() * (S / (8 * n_epochs))
+        _run(3, w, rho, prm, nh, NMAX, True, u1, 100, 2500, mode, g=gm, work=work)
+        work = work.cpu().numpy()
         out[form] = {"mag_evals_per_s_fwd_vjp": S / dt, "ms": dt * 1e3,
-                     "unconverged": int((res[form][1] == 0).sum().item()),
+                     "eager_ms_first_call": eager_s * 1e3, "graph_replay_matches_eager": same,
+                     "unconverged": int((captured[1] == 0).sum().item()),
+                     "work": {"root_iterations_fp64": int(work[0]),
+                              "root_iterations_fp32": int(work[1]),
+                              "root_iterations_comp": int(work[2]),
+                              "limb_solves": int(work[3]), "contour_points": int(work[4])},
                      "roofline": _roofline_entry(contour_flops(work, 3, True, vjp=True,
-                                                               cf32=form == "cf32"), dt),
-                     "roofline_note": "work counters from the first 8 parameter sets, scaled"}
+                                                               cf32=form == "cf32"), dt)}
+        res[form] = (captured[0].clone(), captured[1].clone(), captured[2].clone())
+        del graph, captured
     m64, m32 = res["fp64"][0], res["cf32"][0]
     e = ((m32 - m64).abs() / m64.abs()).cpu().numpy()
     out["cf32"].update({"max_rel_err_vs_fp64": float(e.max()),
                         "p99_rel_err_vs_fp64": float(np.quantile(e, 0.99)),
                         "frac_le_1e-8": float(np.mean(e <= 1e-8))})
     if cpu_s > 0:
+        def kw(kk, e):
+            return dict(w0=complex(b["w"][kk][e]), rho=float(b["rho"][kk]), nlenses=3,
+                        npts_limb=int(npts_all[kk][e]), limb_darkening=True,
+                        u1=float(b["u1"][kk]), npts_ld=100, s=float(b["s"][kk]),
+                        q=float(b["q"][kk]), q3=float(b["q3"][kk]), r3=float(b["r3"][kk]),
+                        psi=float(b["psi"][kk]))
         # every 25th epoch of the parameter sets in order, so the sample spans whole
         # light curves (caustic crossings included)
-        items = (dict(w0=complex(b["w"][k][e]), rho=float(b["rho"][k]), nlenses=3,
-                      npts_limb=npts, limb_darkening=True, u1=float(b["u1"][k]), npts_ld=100,
-                      s=float(b["s"][k]), q=float(b["q"][k]), q3=float(b["q3"][k]),
-                      r3=float(b["r3"][k]), psi=float(b["psi"][k]))
-                 for k in range(n_sets) for e in range(0, n_epochs, 25))
-        out["cpu_baseline"] = cpu_rates(_oracle_extended, list(items)[:20000], cpu_s,
+        items = [kw(kk, e) for kk in range(n_sets) for e in range(0, n_epochs, 25)][:20000]
+        out["cpu_baseline"] = cpu_rates(_oracle_extended, items, cpu_s,
                                         "C5 epochs (every 25th epoch of the light curves in "
-                                        "order), forward only", "mag evals/s (forward only)")
+                                        "order, their npts_limb), forward only",
+                                        "mag evals/s (forward only)")
         # error of the FP64 GPU form against the oracle on a sample around and away from
         # the caustic crossings: the reference's route as such, and with the canonical
         # root polish that the GPU path applies (where the reference's plain stopping
         # test leaves ill-conditioned images off by enough to flip the 1e-6 filter)
-        rows = [(kk, e) for kk in range(min(k, 32)) for e in (300, 744, 749, 753, 1200)]
-        samp = [dict(w0=complex(b["w"][kk][e]), rho=float(b["rho"][kk]), nlenses=3,
-                     npts_limb=npts, limb_darkening=True, u1=float(b["u1"][kk]), npts_ld=100,
-                     s=float(b["s"][kk]), q=float(b["q"][kk]), q3=float(b["q3"][kk]),
-                     r3=float(b["r3"][kk]), psi=float(b["psi"][kk])) for kk, e in rows]
+        nl = min(k, 16)
+        rows = [(kk, e) for kk in range(nl) for e in (300, 744, 749, 753, 1200)]
+        samp = [kw(kk, e) for kk, e in rows]
         import multiprocessing as mp
         with mp.get_context("spawn").Pool(host_cores()[0], initializer=_pool_init,
                                           initargs=(ROOT,)) as pool:
@@ -748,7 +779,8 @@ def bench_triple(steps, warmup, n_sets=1024, n_epochs=1500, npts=256, budget_s=4
         eo_t = np.abs(pairs[:, 0] - tq) / tq
         out["fp64"]["vs_oracle"] = {
             "epochs": len(rows), "sample": "epochs 300, 744, 749, 753, 1200 of the first "
-                                           f"{min(k, 32)} light curves (crossing near 749.5)",
+                                           f"{nl} light curves (crossing near 749.5), their "
+                                           "npts_limb",
             "frac_le_1e-10": float(np.mean(e_p <= 1e-10)),
             "frac_le_1e-8": float(np.mean(e_p <= 1e-8)),
             "max_rel_err": float(e_p.max()), "p99_rel_err": float(np.quantile(e_p, 0.99)),

@@ -42,6 +42,7 @@ SIGNATURES = {
     "caus_gpu_ehrlich_aberth_status": (ctypes.c_int, []),
     "caus_gpu_ehrlich_aberth_vjp_void": (None, [_vp, ctypes.POINTER(_vp), ctypes.c_char_p,
                                                 ctypes.c_size_t]),
+    "caus_ext_set_flat_blocks_per_cu": (ctypes.c_int, [_i32]),
     "caus_roots_vjp": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i64, _i32]),
     "caus_mag_point_source": (ctypes.c_int, [_vp, _i32, _vp, _i64, _vp, _i64, _vp, _vp, _i32,
                                              _i32]),

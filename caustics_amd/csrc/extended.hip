@@ -1,4 +1,6 @@
 // C-ABI entry points of the extended-source contour kernels.
+#include <atomic>
+
 #include "extended_impl.hpp"
 
 extern "C" int caus_mag_extended_source_vjp(void*, int32_t, const double*, int64_t, const double*,
@@ -22,8 +24,19 @@ extern "C" int64_t caus_extended_workspace_bytes(int32_t nlenses, int64_t S, int
 }
 
 namespace caus {
-// grid of the flat extended-source kernels: enough lanes for the work list, at most a
-// few resident waves per SIMD (the kernels stride over their work list)
+// workgroups per CU of the flat (grid-stride) kernels; set by the launch-schedule tuner
+static std::atomic<int> g_flat_per_cu{8};
+}  // namespace caus
+
+extern "C" int caus_ext_set_flat_blocks_per_cu(int32_t n) {
+    if (n < 1 || n > 64) return CAUS_ERR_ARG;
+    caus::g_flat_per_cu.store(n, std::memory_order_relaxed);
+    return CAUS_OK;
+}
+
+namespace caus {
+// grid of the flat extended-source kernels: enough lanes for the work list, at most
+// g_flat_per_cu workgroups per CU (the kernels stride over their work list)
 int ext_flat_grid(int64_t max_lanes) {
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess) {
@@ -33,7 +46,7 @@ int ext_flat_grid(int64_t max_lanes) {
             cus = c;
     }
     const int64_t need = (max_lanes + kExtFlatBlock - 1) / kExtFlatBlock;
-    const int64_t cap = (int64_t)cus * 8;
+    const int64_t cap = (int64_t)cus * g_flat_per_cu.load(std::memory_order_relaxed);
     return (int)(need < 1 ? 1 : (need < cap ? need : cap));
 }
 }  // namespace caus

@@ -86,8 +86,11 @@ def _ragged_layout(npts_host, S, npts_max, need_npts):
     return hit
 
 
+_flat_set = [None]
+
+
 def _run(nlenses, w, rho_t, prm, npts_host, npts_max, ld, u1_t, npts_ld, itmax, mode_id, g=None,
-         max_chunk=None, work=None, streams=2, skip=None):
+         max_chunk=None, work=None, streams=None, skip=None, chunk_bytes=None):
     """One or more launch sequences of the contour pipeline, one per chunk of sources.
     A ragged batch (npts_host, per-source npts_limb held on the host) is processed in
     descending npts_limb order, so that every launch holds sources of similar cost (no
@@ -101,9 +104,21 @@ def _run(nlenses, w, rho_t, prm, npts_host, npts_max, ld, u1_t, npts_ld, itmax, 
     is applied on the device, so nothing waits for the host.  Per-source results do not
     depend on the batch around them.  `work`, a (5,) int64 CUDA tensor, accumulates the
     kernels' work counters (FP64, FP32 and compensated root-iterations, limb-point
-    solves, contour points)."""
+    solves, contour points).  The schedule (chunk workspace budget `chunk_bytes`,
+    `streams`, the flat kernels' workgroups per CU) comes from the tuned table of
+    caustics_amd.autotune unless the caller fixes it; it changes no result."""
     L = _lib.lib()
     S = w.numel()
+    if max_chunk is None and (chunk_bytes is None or streams is None):
+        from . import autotune
+        sch = autotune.lookup(nlenses, ld, g is not None, npts_max)
+        chunk_bytes = chunk_bytes or sch["chunk_bytes"]
+        streams = streams or sch["streams"]
+        if _flat_set[0] != sch["flat_per_cu"]:
+            _lib.check(L.caus_ext_set_flat_blocks_per_cu(int(sch["flat_per_cu"])), "schedule")
+            _flat_set[0] = sch["flat_per_cu"]
+    chunk_bytes = chunk_bytes or (2 << 30)
+    streams = streams or 2
     P = 0 if prm is None else prm.shape[1]
     if extended_workspace_bytes(nlenses, 1, npts_max) < 0:
         raise ValueError(f"npts_limb={npts_max} out of range (4..4096)")
@@ -123,7 +138,7 @@ def _run(nlenses, w, rho_t, prm, npts_host, npts_max, ld, u1_t, npts_ld, itmax, 
     while lo < S:
         nm = npts_max if eff is None else int(eff[lo])
         per = extended_workspace_bytes(nlenses, 1, nm)
-        cnt = max_chunk or max(1, (2 << 30) // per)
+        cnt = max_chunk or max(1, int(chunk_bytes) // per)
         hi = min(S, lo + cnt)
         chunks.append((lo, hi, nm))
         lo = hi

@@ -202,3 +202,14 @@ def triple_caustic_lightcurves(n_sets, n_epochs, seed=5, u1=0.4):
     t = np.linspace(-2 * 20.0, 2 * 20.0, n_epochs)
     w = trajectory(t, t0, u0, tE, alpha)    # (n_sets, n_epochs)
     return dict(w=w, rho=rho, u1=np.full(n_sets, u1), t=t, **p)
+
+
+def crossing_npts(b, lo=256, hi=4096):
+    """Uneven limb resolutions for triple_caustic_lightcurves: hi within one source radius
+    of the caustic point the trajectories cross at t = 0, halved per doubling of the
+    distance (in source radii) down to lo beyond 8 radii (the adaptive resolution an
+    inference run would give epochs near a crossing).  Returns (n_sets, n_epochs) int32."""
+    tE = 20.0
+    x = np.abs(np.asarray(b["t"]))[None, :] / tE / np.asarray(b["rho"])[:, None]
+    k = np.clip(np.ceil(np.log2(np.maximum(x, 1e-300))), 0, None)
+    return np.clip(hi / 2.0 ** k, lo, hi).astype(np.int32)

@@ -193,6 +193,14 @@ int caus_hexadecapole_vjp(void* stream, int32_t nlenses, const double* w, int64_
  *   workspace: caus_extended_workspace_bytes(nlenses, S, npts_max) bytes of device memory
  * mode CAUS_MODE_FP64 or CAUS_MODE_CF32. */
 int64_t caus_extended_workspace_bytes(int32_t nlenses, int64_t S, int32_t npts_max);
+
+/* Launch schedule of the contour pipeline's grid-stride kernels (limb solves, added
+ * points): at most n workgroups of 256 lanes per CU (default 8, 1 <= n <= 64).  No
+ * reference counterpart (the reference maps over sources sequentially,
+ * lightcurve.py:245-254); set per workload by caustics_amd.autotune from its tuned
+ * table.  Process-wide; read when a batch is launched (a captured graph keeps the
+ * grid it was captured with).  Results do not depend on it. */
+int caus_ext_set_flat_blocks_per_cu(int32_t n);
 int caus_mag_extended_source(void* stream, int32_t nlenses, const double* w0, int64_t S,
                              const double* rho, const double* params, int64_t params_stride,
                              const int32_t* npts_limb, int32_t npts_max, int32_t limb_darkening,

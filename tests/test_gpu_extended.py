@@ -160,6 +160,25 @@ def test_chunks_and_streams_do_not_change_results(gpu):
     torch.cuda.synchronize()
     for a, b in zip(one, many):
         assert torch.equal(a, b)
+    # the tuner's knobs: workspace budget per chunk, three streams, and the flat kernels'
+    # workgroups per CU (4 and 16), against the tuned table's schedule
+    from caustics_amd import extended_source as X
+    L = _lib.lib()
+    try:
+        for flat, cb, ns in ((4, 64 << 20, 3), (16, 8 << 30, 1)):
+            _lib.check(L.caus_ext_set_flat_blocks_per_cu(flat), "schedule")
+            X._flat_set[0] = flat
+            alt = _run(*args, g=gm, streams=ns, chunk_bytes=cb)
+            torch.cuda.synchronize()
+            for a, b in zip(one, alt):
+                assert torch.equal(a, b), (flat, cb, ns)
+    finally:
+        L.caus_ext_set_flat_blocks_per_cu(8)
+        X._flat_set[0] = 8
+    tuned = _run(*args, g=gm)
+    for a, b in zip(one, tuned):
+        assert torch.equal(a, b)
+    assert L.caus_ext_set_flat_blocks_per_cu(0) == _lib.CAUS_ERR_ARG
 
 
 def test_ragged_batch_matches_individual(gpu):
