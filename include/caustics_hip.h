@@ -183,7 +183,9 @@ int caus_hexadecapole_vjp(void* stream, int32_t nlenses, const double* w, int64_
  *   w0 (S,) complex source centres; rho (S,); params rows as for caus_mag_point_source
  *   npts_limb (S,) int32 per-source limb resolution (the reference's npts_limb) or NULL to
  *            use npts_max for every source; each value must lie in [4, npts_max]
- *            (out-of-range values are clamped and flagged in status)
+ *            (out-of-range values are clamped and flagged in status), or be 0: the source
+ *            is skipped (no work; mag 0, gradient 0, status 1), so a caller can leave
+ *            sources out on the device without reordering the batch
  *   npts_max  largest npts_limb, <= 4096 (sets the workspace size)
  *   limb_darkening 0: uniform source (trapezoid Green integral); 1: linear limb darkening
  *            with coefficient u1[i*u1_stride] and npts_ld Gauss-Legendre points per line
@@ -191,8 +193,17 @@ int caus_hexadecapole_vjp(void* stream, int32_t nlenses, const double* w, int64_
  *   mag (S,) output; status (S,) uint8 or NULL (0: a root solve did not converge or
  *            npts_limb was out of range)
  *   workspace: caus_extended_workspace_bytes(nlenses, S, npts_max) bytes of device memory
- * mode CAUS_MODE_FP64 or CAUS_MODE_CF32. */
+ * mode CAUS_MODE_FP64, CAUS_MODE_FP64_COMP (the limb solves run the compensated stage,
+ * the reference's roots_compensated=True, extended_source.py:750,839) or CAUS_MODE_CF32.
+ * Every limb solve ends with a compensated polish to the doubles nearest the exact
+ * roots, so the three modes see the same images. */
 int64_t caus_extended_workspace_bytes(int32_t nlenses, int64_t S, int32_t npts_max);
+int caus_mag_extended_source(void* stream, int32_t nlenses, const double* w0, int64_t S,
+                             const double* rho, const double* params, int64_t params_stride,
+                             const int32_t* npts_limb, int32_t npts_max, int32_t limb_darkening,
+                             const double* u1, int64_t u1_stride, int32_t npts_ld, double* mag,
+                             uint8_t* status, void* workspace, int64_t workspace_bytes,
+                             int32_t itmax, int32_t mode);
 
 /* Launch schedule of the contour pipeline's grid-stride kernels (limb solves, added
  * points): at most n workgroups of 256 lanes per CU (default 8, 1 <= n <= 64).  No
@@ -201,12 +212,6 @@ int64_t caus_extended_workspace_bytes(int32_t nlenses, int64_t S, int32_t npts_m
  * table.  Process-wide; read when a batch is launched (a captured graph keeps the
  * grid it was captured with).  Results do not depend on it. */
 int caus_ext_set_flat_blocks_per_cu(int32_t n);
-int caus_mag_extended_source(void* stream, int32_t nlenses, const double* w0, int64_t S,
-                             const double* rho, const double* params, int64_t params_stride,
-                             const int32_t* npts_limb, int32_t npts_max, int32_t limb_darkening,
-                             const double* u1, int64_t u1_stride, int32_t npts_ld, double* mag,
-                             uint8_t* status, void* workspace, int64_t workspace_bytes,
-                             int32_t itmax, int32_t mode);
 
 /* Forward + VJP of caus_mag_extended_source in one pass.  For cotangents g_mag (S,) it
  * writes g_params (S, 4+P): g * dmag/d(Re w0, Im w0, rho, u1, lens params...), the
