@@ -469,46 +469,54 @@ __device__ __forceinline__ bool fast_step(const cd (&p)[DEG + 1], const double (
     return true;
 }
 
-// Plain FP64 step for root j with the reference's branch structure (RefCompat form):
-// |z| > 1 evaluates the reversed polynomial at 1/z and corrects with z^2 = exp(2 log z)
-// (ehrlich_aberth.h:102-143, cpow.h:36-43).  Both branches run the same arithmetic as
-// the reference's, but as one code path whose coefficients, reciprocal and bound are
-// picked per lane by selects, so that a wave whose lanes sit on both sides of |z| = 1
-// does not run the Horner passes twice; only exp(2 log z) stays in a branch.
+// Plain FP64 step for root j with the reference's branch structure (RefCompat form).
 template <int DEG, bool GUARD = false>
 __device__ __forceinline__ bool ref_step(const cd (&p)[DEG + 1], const double (&al)[DEG + 1],
                                          cd (&z)[DEG], int j, int& nrev) {
-    const cd zj = z[j];
-    const double az = GUARD ? cabs_safe(zj) : cabs(zj);
-    const bool rev = az > 1;
-    nrev += rev ? 1 : 0;
-    const double ax = rev ? rcp(az) : az;
-    const cd xr = GUARD ? crcp_scaled(zj) : crcp<2>(zj);
-    const cd x = rev ? xr : zj;
-    double b = rev ? al[0] : al[DEG];
+    cd zj = z[j];
+    double az = GUARD ? cabs_safe(zj) : cabs(zj);
+    double b;
+    cd h, hd = cd{0.0, 0.0};
+    if (az > 1) {
+        ++nrev;
+        double ax = rcp(az);
+        cd x = GUARD ? crcp_scaled(zj) : crcp<2>(zj);
+        b = al[0];
 #pragma unroll
-    for (int t = 1; t <= DEG; ++t) b = fma(b, ax, rev ? al[t] : al[DEG - t]);
-    cd h = rev ? p[0] : p[DEG], hd = cd{0.0, 0.0};
+        for (int k = 1; k <= DEG; ++k) b = fma(b, ax, al[k]);
+        h = p[0];
 #pragma unroll
-    for (int t = 1; t <= DEG; ++t) {
-        hd = cfma(hd, x, h);
-        h = cfma(h, x, rev ? p[t] : p[DEG - t]);
-    }
-    if (GUARD && !rev && !(EA_EPS * b >= 0x1p-480))
-        return scaled_step<DEG>(z, j, h, hd, EA_EPS * b);
-    const double ah = (GUARD && rev) ? cabs_safe(h) : cabs(h);
-    if (!(ah > EA_EPS * b)) return true;
-    const cd S = aberth_sum<DEG>(z, j, zj);
-    cd num = h, den;
-    if (rev) {
-        const cd z2 = pow2_explog(zj);
-        num = z2 * h;
-        den = ((double)DEG * zj) * h - hd - num * S;
+        for (int k = 1; k <= DEG; ++k) {
+            hd = cfma(hd, x, h);
+            h = cfma(h, x, p[k]);
+        }
+        if ((GUARD ? cabs_safe(h) : cabs(h)) > EA_EPS * b) {
+            cd S = aberth_sum<DEG>(z, j, zj);
+            cd z2 = pow2_explog(zj);
+            cd num = z2 * h;
+            cd den = ((double)DEG * zj) * h - hd - num * S;
+            z[j] = zj - cdiv<2>(num, den);
+            return false;
+        }
+        return true;
     } else {
-        den = hd - h * S;
+        b = al[DEG];
+#pragma unroll
+        for (int k = DEG - 1; k >= 0; --k) b = fma(b, az, al[k]);
+        h = p[DEG];
+#pragma unroll
+        for (int k = DEG - 1; k >= 0; --k) {
+            hd = cfma(hd, zj, h);
+            h = cfma(h, zj, p[k]);
+        }
+        if (GUARD && !(EA_EPS * b >= 0x1p-480)) return scaled_step<DEG>(z, j, h, hd, EA_EPS * b);
+        if (cabs(h) > EA_EPS * b) {
+            cd S = aberth_sum<DEG>(z, j, zj);
+            z[j] = zj - cdiv<2>(h, hd - h * S);
+            return false;
+        }
+        return true;
     }
-    z[j] = zj - cdiv<2>(num, den);
-    return false;
 }
 
 // Iteration state of one polynomial (registers of one lane).  GUARD: see fast_step.

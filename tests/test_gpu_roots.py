@@ -46,7 +46,6 @@ def test_lens_roots_match_oracle(gpu, nlenses, n, mode, init):
     assert np.max(residual_ratio(c, z)[good]) < 1e-13
     if init == "reference" and mode == "fp64":
         same_order = np.all(zm == z, axis=1)
-        print(f"  root order as the oracle's: {same_order.mean():.5f}")
         assert same_order.mean() > 0.99
 
 
