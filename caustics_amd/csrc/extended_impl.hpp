@@ -69,13 +69,6 @@ struct ExtWS {
     double* yz;
     int32_t* ord[2];   // refinement: limb order -> index of the point in append order
     uint64_t* gk[2];   // refinement: key of the gap after each point in limb order
-    // buffer b of a double-buffered array, chosen by a select: indexing the member
-    // arrays with a runtime b would place the whole struct in scratch memory
-    __device__ __forceinline__ double* theta_(int b) const { return b ? theta[1] : theta[0]; }
-    __device__ __forceinline__ cd* z_(int b) const { return b ? z[1] : z[0]; }
-    __device__ __forceinline__ uint32_t* flg_(int b) const { return b ? flg[1] : flg[0]; }
-    __device__ __forceinline__ int32_t* ord_(int b) const { return b ? ord[1] : ord[0]; }
-    __device__ __forceinline__ uint64_t* gk_(int b) const { return b ? gk[1] : gk[0]; }
 };
 
 __device__ inline ExtWS ext_ws(char* base, int deg, int mmax) {
@@ -169,44 +162,6 @@ __device__ __forceinline__ uint64_t nn_map(const cd (&a)[DEG], const cd (&b)[DEG
         conflict = conflict || ((hit >> bj) & 1u);
         hit |= 1u << bj;
         p |= (uint64_t)bj << (4 * i);
-    }
-    return conflict ? (p | kConflictBit) : p;
-}
-
-// nn_map with point b read one image at a time (j outer): for every row i the first j
-// of least distance, as nn_map finds it, with b never held whole in registers
-template <int DEG>
-__device__ __forceinline__ uint64_t nn_map_stream(const cd (&a)[DEG], const cd* b) {
-    double best[DEG];
-    int bj[DEG];
-    {
-        const cd b0 = b[0];
-#pragma unroll
-        for (int i = 0; i < DEG; ++i) {
-            best[i] = norm2(b0 - a[i]);
-            bj[i] = 0;
-        }
-    }
-#pragma unroll
-    for (int j = 1; j < DEG; ++j) {
-        const cd bv = b[j];
-#pragma unroll
-        for (int i = 0; i < DEG; ++i) {
-            const double d = norm2(bv - a[i]);
-            if (d < best[i]) {
-                best[i] = d;
-                bj[i] = j;
-            }
-        }
-    }
-    unsigned hit = 0;
-    bool conflict = false;
-    uint64_t p = 0;
-#pragma unroll
-    for (int i = 0; i < DEG; ++i) {
-        conflict = conflict || ((hit >> bj[i]) & 1u);
-        hit |= 1u << bj[i];
-        p |= (uint64_t)bj[i] << (4 * i);
     }
     return conflict ? (p | kConflictBit) : p;
 }
@@ -311,12 +266,13 @@ __device__ __forceinline__ void permute_pt(cd (&z)[DEG], uint64_t p) {
 
 // _permute_images over points 0..M-1 of buffer b (extended_source.py:34-53)
 template <int DEG, int NT>
-__device__ __forceinline__ void chain_images(cd* zb, uint32_t* fb, uint64_t* perm, int M) {
+__device__ void chain_images(cd* zb, uint32_t* fb, uint64_t* perm, int M) {
     const int tid = threadIdx.x;
     for (int k = tid; k < M - 1; k += NT) {
-        cd a[DEG];
+        cd a[DEG], b[DEG];
         load_pt<DEG>(zb, k, a);
-        perm[k] = nn_map_stream<DEG>(a, zb + (size_t)(k + 1) * DEG);
+        load_pt<DEG>(zb, k + 1, b);
+        perm[k] = nn_map<DEG>(a, b);
     }
     __syncthreads();
     if (tid < 64) {
@@ -391,17 +347,10 @@ __device__ __forceinline__ void chain_images(cd* zb, uint32_t* fb, uint64_t* per
         // gather straight from memory (row i <- raw row p[i]): no register array with a
         // dynamic index, which the compiler would place in scratch
         const uint64_t p = perm[k];
-        double re[DEG], im[DEG];
-        const double2* src = reinterpret_cast<const double2*>(zb + (size_t)k * DEG);
+        cd z[DEG];
 #pragma unroll
-        for (int i = 0; i < DEG; ++i) {
-            const double2 v = src[pget(p, i)];
-            re[i] = v.x;
-            im[i] = v.y;
-        }
-        double2* dst = reinterpret_cast<double2*>(zb + (size_t)k * DEG);
-#pragma unroll
-        for (int i = 0; i < DEG; ++i) dst[i] = make_double2(re[i], im[i]);
+        for (int i = 0; i < DEG; ++i) z[i] = zb[(size_t)k * DEG + pget(p, i)];
+        store_pt<DEG>(zb, k, z);
         fb[k] = permute_flags<DEG>(fb[k], p);
     }
     __syncthreads();
@@ -1331,39 +1280,7 @@ static __global__ __launch_bounds__(kExtPlanBlock) void k_ext_plan(ExtArgs A) {
 // inside one cluster of nearly double images (near a caustic), where the FP64 stopping
 // test accepts both before they separate; the compensated polish that ends every solve
 // continues the Aberth iteration there and takes them to the cluster's distinct roots.
-// Limb point 0 is solved in the reference's root order by k_ext_limb_first, which runs
-// before this kernel: the lane that holds angle 0 starts its angle 1 from point 0's
-// stored roots, as it would from its own solve, and no lane of this kernel carries the
-// reference-order solver (its registers spilled the whole kernel).
-template <int NL>
-__global__ __launch_bounds__(kExtFlatBlock) void k_ext_limb_first(ExtArgs A) {
-    constexpr int DEG = ext_degree<NL>();
-    const ExtBatch B = ext_batch(A);
-    for (int64_t s = (int64_t)blockIdx.x * kExtFlatBlock + threadIdx.x; s < A.S;
-         s += (int64_t)gridDim.x * kExtFlatBlock) {
-        if (B.npts[s] == 0) continue;   // skipped source
-        const int M0 = ext_npts_init(B.npts[s]);
-        LensFrame<NL> lf;
-        cd w0;
-        double rho;
-        ext_source<NL>(A, s, lf, w0, rho);
-        ExtWS W = ext_ws(ext_source_base<DEG>(A, s), DEG, A.mmax);
-        cd z[DEG];
-        SolveStats st{0, 0, 0, 0};
-        const double th = ext_limb_angle(0, M0 - 1);
-        double sn, cs;
-        sincos(th, &sn, &cs);
-        uint32_t f;
-        if (!ext_solve_point<NL, MODE_FP64>(ext_limb_point(w0, rho, cs, sn), lf, z, f, A.itmax,
-                                            A.work ? &st : nullptr, false, true))
-            B.fail[s] = 1;
-        W.theta[0][0] = th;
-        store_pt<DEG>(W.z[0], 0, z);
-        W.flg[0][0] = f;
-        if (A.work) ext_count(A.work, st, 1);
-    }
-}
-
+// Limb point 0 is solved in the reference's root order (ext_solve_point, first).
 template <int NL, int MODE>
 __global__ __launch_bounds__(kExtFlatBlock)
 __attribute__((amdgpu_waves_per_eu(2)))  // 2 waves/SIMD measured faster than 1
@@ -1389,23 +1306,20 @@ void k_ext_limb_init(ExtArgs A) {
         ExtWS W = ext_ws(ext_source_base<DEG>(A, s), DEG, A.mmax);
         cd z[DEG];
         SolveStats st{0, 0, 0, 0};
-        // point 0 was solved by k_ext_limb_first: its roots start angle 1
-        const int ks = k0 == 0 ? 1 : k0;
-        if (k0 == 0 && ks < k1) load_pt<DEG>(W.z[0], 0, z);
-        for (int k = ks; k < k1; ++k) {
+        for (int k = k0; k < k1; ++k) {
             const double th = ext_limb_angle(k, M0 - 1);
             double sn, cs;
             sincos(th, &sn, &cs);
             uint32_t f;
             // the lane's later angles start from the previous angle's roots
             if (!ext_solve_point<NL, MODE>(ext_limb_point(w0, rho, cs, sn), lf, z, f, A.itmax,
-                                           A.work ? &st : nullptr, k > k0, false))
+                                           A.work ? &st : nullptr, k > k0, k == 0))
                 B.fail[s] = 1;
             W.theta[0][k] = th;
             store_pt<DEG>(W.z[0], k, z);
             W.flg[0][k] = f;
         }
-        if (A.work) ext_count(A.work, st, k1 - ks);
+        if (A.work) ext_count(A.work, st, k1 - k0);
     }
 }
 
@@ -1445,7 +1359,7 @@ __global__ __launch_bounds__(kExtWaveBlock) void k_ext_chain(ExtArgs A, int cur,
         }
     } else {
         const int M = ext_points(B.npts[s], 10);
-        const int32_t* ord = W.ord_(cur);
+        const int32_t* ord = W.ord[cur];
         for (int k = tid; k < M; k += kExtWaveBlock) {
             const int a = ord[k];
 #pragma unroll
@@ -1476,8 +1390,8 @@ __global__ __launch_bounds__(NT) void k_ext_select(ExtArgs A, int cur, int round
     const int M = ext_points(n, round), nadd = ext_n_add(n);
     ExtWS W = ext_ws(ext_source_base<DEG>(A, src), DEG, A.mmax);
     const int nxt = cur ^ 1;
-    const int32_t* ord = W.ord_(cur);
-    const uint64_t* gk = W.gk_(cur);
+    const int32_t* ord = W.ord[cur];
+    const uint64_t* gk = W.gk[cur];
     int P2 = 1;
     while (P2 < M - 1) P2 <<= 1;
     uint64_t* s_key = s_dyn;
@@ -1528,8 +1442,8 @@ __global__ __launch_bounds__(NT) void k_ext_select(ExtArgs A, int cur, int round
     for (int k = tid; k < M; k += NT) {
         const int sh = shift[k];
         const int d = k + (sh & 0x3FFFFFFF);
-        W.ord_(nxt)[d] = ord[k];
-        if (k + 1 < M && !(sh >> 30)) W.gk_(nxt)[d] = gk[k];
+        W.ord[nxt][d] = ord[k];
+        if (k + 1 < M && !(sh >> 30)) W.gk[nxt][d] = gk[k];
     }
     const int64_t t0 = B.off_add[src];
     for (int c = tid; c < nadd; c += NT) {
@@ -1542,7 +1456,7 @@ __global__ __launch_bounds__(NT) void k_ext_select(ExtArgs A, int cur, int round
         B.t_dst[t0 + c] = d;
         B.t_app[t0 + c] = M + c;
         B.t_th[t0 + c] = 0.5 * (W.theta[0][a] + W.theta[0][b]);
-        W.ord_(nxt)[d] = M + c;
+        W.ord[nxt][d] = M + c;
     }
 }
 
@@ -1594,10 +1508,10 @@ void k_ext_solve_add(ExtArgs A, int cur) {
         cd zn[DEG];
 #pragma unroll
         for (int i = 0; i < DEG; ++i) zn[i] = z[pget(p, i)];
-        W.gk_(nxt)[d - 1] = gap_key<DEG>(left, W.flg[0][a], zn, f);
+        W.gk[nxt][d - 1] = gap_key<DEG>(left, W.flg[0][a], zn, f);
         cd right[DEG];
         load_pt<DEG>(W.z[0], b, right);
-        W.gk_(nxt)[d] = gap_key<DEG>(zn, f, right, W.flg[0][b]);
+        W.gk[nxt][d] = gap_key<DEG>(zn, f, right, W.flg[0][b]);
     }
 }
 
@@ -1622,8 +1536,8 @@ __global__ __launch_bounds__(kExtWaveBlock) void k_ext_contours(ExtArgs A, int b
     const int M = ext_points(B.npts[src], 10);
     ExtWS W = ext_ws(ext_source_base<DEG>(A, src), DEG, A.mmax);
     ExtContours* C = ext_contours<DEG>(A, src);
-    const cd* zb = W.z_(buf);
-    const uint32_t* fb = W.flg_(buf);
+    const cd* zb = W.z[buf];
+    const uint32_t* fb = W.flg[buf];
     // closure test (extended_source.py:289-291)
     if (tid == 0) s_allmask = 0xFFFFu;
     if (tid < DEG) s_nst[tid] = s_nen[tid] = 0;
@@ -1944,9 +1858,9 @@ void k_ext_integrate(ExtArgs A, int buf) {
         gauss_legendre(g.n2, s_gl + 256, s_gl + 384);
     }
     __syncthreads();
-    const cd* zb = W.z_(buf);
-    const uint32_t* fb = W.flg_(buf);
-    const double* thb = W.theta_(buf);
+    const cd* zb = W.z[buf];
+    const uint32_t* fb = W.flg[buf];
+    const double* thb = W.theta[buf];
     // the source's contours (extended_source.py:860-899): NL == 1 both images (tail index
     // N-1, :860-866); otherwise every closed image row, then the three stitched contours
     __shared__ ExtCEntry s_ct[kExtMaxContours];
@@ -2036,8 +1950,6 @@ int ext_run(hipStream_t s, const ExtArgs& a) {
     const int64_t S = a.S;
     const unsigned nb = (unsigned)S;
     hipLaunchKernelGGL(k_ext_plan, dim3(1), dim3(kExtPlanBlock), 0, s, a);
-    hipLaunchKernelGGL((k_ext_limb_first<NL>), dim3(ext_flat_grid(S)), dim3(kExtFlatBlock), 0, s,
-                       a);
     const int64_t lanes_init = S * (int64_t)((ext_npts_init(a.npts_def) + kExtPer - 1) / kExtPer);
     hipLaunchKernelGGL((k_ext_limb_init<NL, MODE>), dim3(ext_flat_grid(lanes_init)),
                        dim3(kExtFlatBlock), 0, s, a);
