@@ -370,11 +370,12 @@ def bench_point_source(steps, warmup, cpu_s=0.0):
     out["vs_oracle"] = {"max_rel_err": float(e.max()), "p99_rel_err": float(np.quantile(e, 0.99)),
                         "frac_le_1e-10": float(np.mean(e <= 1e-10))}
     if cpu_s > 0:
+        from oracle import lens
         wc = w.cpu().numpy()
-        chunks = [(wc[i:i + 2000], dict(s=0.9, q=0.1)) for i in range(0, n, 2000)] * 400
-        out["cpu_baseline"] = cpu_rates(_oracle_point_source, chunks, cpu_s,
-                                        "point-source magnifications of the same 10^4 positions",
-                                        "mag evals/s", per_item=2000)
+        out["cpu_baseline"] = cpu_rates_omp(
+            lambda nt: lens.mag_point_source_c(wc, 2, nthreads=nt, s=0.9, q=0.1), n, cpu_s,
+            "point-source magnifications of the same 10^4 positions (compiled C oracle, "
+            "oracle/lens_oracle.c)", "mag evals/s")
     return out
 
 
@@ -455,9 +456,13 @@ def bench_lightcurves(steps, warmup, host_check=True, cpu_s=0.0):
                                 "p99_rel_err": float(np.quantile(e, 0.99)),
                                 "frac_le_1e-10": float(np.mean(e <= 1e-10))}
     if cpu_s > 0:
-        out["cpu_baseline"] = cpu_rates(_oracle_point_source, curves * 8, cpu_s,
-                                        "light-curve epochs of the same workload, forward only",
-                                        "mag evals/s (forward only)", per_item=T)
+        from oracle import lens
+        wf = wtraj.reshape(-1)
+        sf, qf = np.repeat(prm["s"], T), np.repeat(prm["q"], T)
+        out["cpu_baseline"] = cpu_rates_omp(
+            lambda nt: lens.mag_point_source_c(wf, 2, nthreads=nt, s=sf, q=qf), L_ * T, cpu_s,
+            "light-curve epochs of the same workload, forward only (compiled C oracle, "
+            "oracle/lens_oracle.c)", "mag evals/s (forward only)")
     rel = ((m32 - m64).abs() / m64.abs()).cpu().numpy()
     out["cf32"]["max_rel_err_vs_fp64"] = float(rel.max())
     out["cf32"]["p99_rel_err_vs_fp64"] = float(np.quantile(rel, 0.99))
@@ -558,6 +563,32 @@ def _oracle_point_source(args):
     from oracle import lens
     w, prm = args
     return lens.mag_point_source(w, 2, **prm)
+
+
+def cpu_rates_omp(call, n_evals, budget_s, what, unit):
+    """Evaluations per second of call(nthreads) (n_evals per call) on one OpenMP thread
+    and on every core of the process, each repeated for about budget_s."""
+    ncores, how = host_cores()
+    out = None
+    for nt in (1, ncores):
+        call(nt)   # warm: thread pool, caches
+        n, t0 = 0, time.perf_counter()
+        while True:
+            call(nt)
+            n += 1
+            if time.perf_counter() - t0 > budget_s:
+                break
+        dt = time.perf_counter() - t0
+        ent = {"value": n * n_evals / dt, "unit": unit, "cores": nt, "kind": "port",
+               "sample": f"{n} x {n_evals} {what}, {dt:.1f} s, {nt} OpenMP thread(s)"
+                         + ("" if nt == 1 else f" ({how})")}
+        if out is None:
+            out = ent
+        else:
+            out["all_cores"] = ent
+    print(f"bench: CPU {what}: 1 core {out['value']:.4g}, {ncores} cores "
+          f"{out['all_cores']['value']:.4g} {unit}", file=sys.stderr, flush=True)
+    return out
 
 
 def cpu_rates(fn, items, budget_s, what, unit, per_item=1):

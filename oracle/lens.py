@@ -241,6 +241,27 @@ def mag_point_source(w, nlenses=2, itmax=2500, compensated=False, init_mode=0, p
     return ((1.0 / np.abs(det)) * mask).sum(axis=0)
 
 
+def mag_point_source_c(w, nlenses=2, itmax=2500, nthreads=1, **params):
+    """mag_point_source in compiled C (lens_oracle.c: ora_mag_point_source) with OpenMP
+    over positions: the same steps, for the host CPU baseline.  Parameters broadcast
+    against w."""
+    import ctypes
+
+    w = np.ascontiguousarray(np.asarray(w, complex).reshape(-1))
+    names = ("s", "q") if nlenses == 2 else ("s", "q", "q3", "r3", "psi")
+    prm = np.ascontiguousarray(np.stack([np.broadcast_to(np.asarray(params[k], float), w.shape)
+                                         for k in names], -1))
+    out = np.empty(w.shape, float)
+    L = ea.lib()
+    f = L.ora_mag_point_source
+    f.restype = ctypes.c_int64
+    f.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                  ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    f(nlenses, w.size, w.ctypes.data, prm.ctypes.data, prm.shape[1], out.ctypes.data, itmax,
+      nthreads)
+    return out
+
+
 # ----------------------------------------------------------------------------
 # gradient oracle: the reference's route (implicit root derivative of the
 # polynomial, ehrlich_aberth_primitive.py:305-319) chained through the
