@@ -536,15 +536,6 @@ def _pool_init(root):
         pass
 
 
-def _pool_noop(x):
-    return x
-
-
-def _oracle_extended(kw):
-    from oracle import extended as OE
-    return OE.mag_extended_source(**kw)
-
-
 def _oracle_extended_pair(kw):
     """(oracle as the reference computes it, oracle with the canonical root polish)"""
     from oracle import extended as OE
@@ -591,41 +582,51 @@ def cpu_rates_omp(call, n_evals, budget_s, what, unit):
     return out
 
 
-def cpu_rates(fn, items, budget_s, what, unit, per_item=1):
-    """Evaluations per second of fn over items on the host: one core (this process, one
-    OpenMP thread) and every core this process may use (os.sched_getaffinity; spawned
-    worker processes of one thread each), each bounded by budget_s of work."""
-    import multiprocessing as mp
-
-    items = list(items)
-    _pool_init(ROOT)
-    n, t0 = 0, time.perf_counter()
-    for it in items:
-        fn(it)
-        n += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    dt1 = time.perf_counter() - t0
-    one = {"value": n * per_item / dt1, "unit": unit, "cores": 1, "kind": "port",
-           "sample": f"{n * per_item} {what}, {dt1:.1f} s"}
+def cpu_rates_omp_items(call, n_items, budget_s, what, unit):
+    """Evaluations per second of call(lo, hi, nthreads), which evaluates items lo..hi-1 of
+    a fixed list, on one OpenMP thread and on every core of the process.  Each thread
+    count walks the list from its start in slices that double from 8 items (at least 4
+    per thread) until budget_s is spent, so both rates cover the same leading items."""
     ncores, how = host_cores()
-    ctx = mp.get_context("spawn")   # fresh interpreters: the workers never touch the GPU
-    with ctx.Pool(ncores, initializer=_pool_init, initargs=(ROOT,)) as pool:
-        pool.map(_pool_noop, range(4 * ncores))   # workers up before the clock starts
-        n, t0 = 0, time.perf_counter()
-        for _ in pool.imap_unordered(fn, items, chunksize=1):
-            n += 1
+    out = None
+    for nt in (1, ncores):
+        call(0, min(n_items, nt), nt)   # warm: thread pool, caches
+        lo, size, t0 = 0, max(8, 4 * nt), time.perf_counter()
+        while lo < n_items:
+            hi = min(n_items, lo + size)
+            call(lo, hi, nt)
+            lo, size = hi, 2 * size
             if time.perf_counter() - t0 > budget_s:
                 break
         dt = time.perf_counter() - t0
-        pool.terminate()
-    allc = {"value": n * per_item / dt, "unit": unit, "cores": ncores, "kind": "port",
-            "sample": f"{n * per_item} {what}, {dt:.1f} s, {ncores} worker processes "
-                      f"({how})"}
-    print(f"bench: CPU {what}: 1 core {one['value']:.4g}, {ncores} cores {allc['value']:.4g} "
-          f"{unit}", file=sys.stderr, flush=True)
-    one["all_cores"] = allc
-    return one
+        ent = {"value": lo / dt, "unit": unit, "cores": nt, "kind": "port",
+               "sample": f"the first {lo} {what}, {dt:.1f} s, {nt} OpenMP thread(s)"
+                         + ("" if nt == 1 else f" ({how})")}
+        if out is None:
+            out = ent
+        else:
+            out["all_cores"] = ent
+    print(f"bench: CPU {what}: 1 core {out['value']:.4g}, {ncores} cores "
+          f"{out['all_cores']['value']:.4g} {unit}", file=sys.stderr, flush=True)
+    return out
+
+
+def _extended_c_call(items):
+    """call(lo, hi, nthreads) over a list of mag_extended_source keyword sets with one lens
+    count: the compiled oracle on items lo..hi-1"""
+    from oracle import extended as OE
+    names = ("s", "q", "q3", "r3", "psi")
+    cols = {k: np.array([it[k] for it in items]) for k in ("w0", "rho", "npts_limb", "u1")}
+    prm = {k: np.array([it[k] for it in items]) for k in names if k in items[0]}
+    nl = items[0]["nlenses"]
+
+    def call(lo, hi, nt):
+        m, bad = OE.mag_extended_source_c(
+            cols["w0"][lo:hi], cols["rho"][lo:hi], nl, npts_limb=cols["npts_limb"][lo:hi],
+            limb_darkening=True, u1=cols["u1"][lo:hi], npts_ld=100, nthreads=nt,
+            **{k: v[lo:hi] for k, v in prm.items()})
+        return m
+    return call
 
 
 def bench_extended_binary(steps, warmup, n_near=2048, n_far=2048, cpu_s=0.0):
@@ -684,9 +685,11 @@ def bench_extended_binary(steps, warmup, n_near=2048, n_far=2048, cpu_s=0.0):
                       npts_limb=int(b["npts"][i]), limb_darkening=True, u1=float(b["u1"][i]),
                       npts_ld=100, s=float(b["s"][i]), q=float(b["q"][i]))
                  for i in range(n_near))
-        out["cpu_baseline"] = cpu_rates(_oracle_extended, list(items), cpu_s,
-                                        "near-caustic C4 sources (their ragged npts_limb), "
-                                        "forward only", "mag evals/s (forward only)")
+        items = list(items)
+        out["cpu_baseline"] = cpu_rates_omp_items(
+            _extended_c_call(items), len(items), cpu_s,
+            "near-caustic C4 sources (their ragged npts_limb), forward only (compiled C++ "
+            "oracle, oracle/extended_oracle.cpp)", "mag evals/s (forward only)")
     return out
 
 
@@ -784,10 +787,11 @@ def bench_triple(steps, warmup, n_sets=1024, n_epochs=1500, budget_s=40.0, cpu_s
         # every 25th epoch of the parameter sets in order, so the sample spans whole
         # light curves (caustic crossings included)
         items = [kw(kk, e) for kk in range(n_sets) for e in range(0, n_epochs, 25)][:20000]
-        out["cpu_baseline"] = cpu_rates(_oracle_extended, items, cpu_s,
-                                        "C5 epochs (every 25th epoch of the light curves in "
-                                        "order, their npts_limb), forward only",
-                                        "mag evals/s (forward only)")
+        out["cpu_baseline"] = cpu_rates_omp_items(
+            _extended_c_call(items), len(items), cpu_s,
+            "C5 epochs (every 25th epoch of the light curves in order, their npts_limb), "
+            "forward only (compiled C++ oracle, oracle/extended_oracle.cpp)",
+            "mag evals/s (forward only)")
         # error of the FP64 GPU form against the oracle on a sample around and away from
         # the caustic crossings: the reference's route as such, and with the canonical
         # root polish that the GPU path applies (where the reference's plain stopping

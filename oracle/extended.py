@@ -15,6 +15,8 @@ Follows (fbartolic/caustics, src/caustics/):
   mag_extended_source .......... extended_source.py:730-901
   _integrate_unif / _integrate_ld / _brightness_profile / trapz_zero_avoiding
                                  integrate.py:18-121, utils.py:43-100
+  mag_extended_source_c ........ the same steps in compiled C++ (extended_oracle.cpp),
+                                 OpenMP over sources: the host CPU baseline
   mag_disk_quadrature .......... not a restatement: the same magnification as a
                                  source-plane integral of mag_point_source, the
                                  truth the contour route is judged against
@@ -424,6 +426,44 @@ def mag_extended_source(w0, rho, nlenses=2, npts_limb=150, limb_darkening=False,
     if return_parts:
         return mag, dict(z=z, mask=mask, par=par, theta=theta, all_closed=all_closed)
     return mag
+
+
+def mag_extended_source_c(w0, rho, nlenses=2, npts_limb=150, limb_darkening=False, u1=0.0,
+                          npts_ld=100, nthreads=1, **params):
+    """mag_extended_source in compiled C++ (extended_oracle.cpp: ora_mag_extended_source)
+    with OpenMP over sources: the same steps, for the host CPU baseline.  w0, rho,
+    npts_limb, u1 and the lens parameters broadcast against each other; returns the
+    magnifications and the number of sources with an unconverged limb solve."""
+    import ctypes
+
+    from . import ea
+
+    names = ("s", "q") if nlenses == 2 else ("s", "q", "q3", "r3", "psi")
+    arrs = np.broadcast_arrays(np.asarray(w0, complex), np.asarray(rho, float),
+                               np.asarray(npts_limb), np.asarray(u1, float),
+                               *[np.asarray(params[k], float) for k in names])
+    shape = arrs[0].shape
+    w = np.ascontiguousarray(arrs[0].reshape(-1))
+    r = np.ascontiguousarray(arrs[1].reshape(-1))
+    npts = np.ascontiguousarray(arrs[2].reshape(-1), dtype=np.int32)
+    u = np.ascontiguousarray(arrs[3].reshape(-1))
+    prm = np.ascontiguousarray(np.stack([a.reshape(-1) for a in arrs[4:]], -1))
+    n1 = int(npts_ld / 2)
+    x1, w1 = np.polynomial.legendre.leggauss(n1)
+    x2, w2 = np.polynomial.legendre.leggauss(npts_ld - n1)
+    out = np.empty(w.shape, float)
+    f = ea.lib().ora_mag_extended_source
+    f.restype = ctypes.c_int64
+    vp, i32 = ctypes.c_void_p, ctypes.c_int
+    f.argtypes = [i32, ctypes.c_int64, vp, vp, vp, ctypes.c_int64, vp, i32, vp, vp, vp, i32, vp,
+                  vp, i32, vp, i32]
+    bad = f(nlenses, w.size, w.ctypes.data, r.ctypes.data, prm.ctypes.data, prm.shape[1],
+            npts.ctypes.data, int(bool(limb_darkening)), u.ctypes.data, x1.ctypes.data,
+            w1.ctypes.data, n1, x2.ctypes.data, w2.ctypes.data, npts_ld - n1, out.ctypes.data,
+            nthreads)
+    if bad < 0:
+        raise ValueError("nlenses must be 2 or 3")
+    return out.reshape(shape), int(bad)
 
 
 def mag_disk_quadrature(w0, rho, nlenses=2, u1=0.0, nr=100, **params):
