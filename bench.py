@@ -690,7 +690,43 @@ def bench_extended_binary(steps, warmup, n_near=2048, n_far=2048, cpu_s=0.0):
             _extended_c_call(items), len(items), cpu_s,
             "near-caustic C4 sources (their ragged npts_limb), forward only (compiled C++ "
             "oracle, oracle/extended_oracle.cpp)", "mag evals/s (forward only)")
+        # error of the FP64 GPU form against the oracle on the sources the gate sent to
+        # the contour route (the first 512 of them, their npts_limb)
+        use64 = res["fp64"][1].cpu().numpy()
+        idx = [i for i in range(n_near) if not use64[i]][:512]
+        mo = _extended_c_call([items[i] for i in idx])(0, len(idx), host_cores()[0])
+        gv = m64.cpu().numpy()[idx]
+        out["fp64"]["vs_oracle"] = _rel_err_stats(gv, mo)
+        out["fp64"]["vs_oracle"]["sample"] = (f"the first {len(idx)} gate-rejected near-caustic "
+                                              "sources, their npts_limb (compiled oracle)")
+        out["fp64"]["vs_oracle"]["largest_differences_vs_truth"] = _truth_judge(
+            gv, mo, [items[i] for i in idx])
     return out
+
+
+def _truth_judge(gpu, ref, kws, top=20):
+    """The largest GPU-oracle differences judged against the source-plane quadrature of
+    the same disk (oracle/extended.py mag_disk_quadrature at 200 nodes, 100 for the
+    resolution gap): which of the two lies closer to it."""
+    import multiprocessing as mp
+    e = np.abs(gpu - ref) / np.abs(ref)
+    idx = np.argsort(-e)[:top]
+    with mp.get_context("spawn").Pool(host_cores()[0], initializer=_pool_init,
+                                      initargs=(ROOT,)) as pool:
+        tr = np.array(pool.map(_oracle_truth, [kws[i] for i in idx], chunksize=1))
+    t = tr[:, 1]
+    eg, eo = np.abs(gpu[idx] - t) / t, np.abs(ref[idx] - t) / t
+    return {"sources": int(len(idx)), "gpu_closer": int((eg < eo).sum()),
+            "oracle_closer": int((eo < eg).sum()), "gpu_median_rel_err": float(np.median(eg)),
+            "oracle_median_rel_err": float(np.median(eo)),
+            "quadrature_resolution_gap_median": float(np.median(np.abs(tr[:, 0] - t) / t))}
+
+
+def _rel_err_stats(gpu, ref):
+    e = np.abs(gpu - ref) / np.abs(ref)
+    return {"n": int(e.size), "max_rel_err": float(e.max()),
+            "p99_rel_err": float(np.quantile(e, 0.99)), "median_rel_err": float(np.median(e)),
+            "frac_le_1e-10": float(np.mean(e <= 1e-10)), "frac_le_1e-8": float(np.mean(e <= 1e-8))}
 
 
 def bench_triple(steps, warmup, n_sets=1024, n_epochs=1500, budget_s=40.0, cpu_s=0.0):
@@ -834,6 +870,18 @@ def bench_triple(steps, warmup, n_sets=1024, n_epochs=1500, budget_s=40.0, cpu_s
                     np.mean(np.abs(g64 - pairs[:, 0]) <= np.abs(pairs[:, 0] - tq))),
                 "where_gpu_differs_gpu_closer": int(((e_p > 1e-10) & (eg_t < eo_t)).sum()),
                 "where_gpu_differs_oracle_closer": int(((e_p > 1e-10) & (eo_t < eg_t)).sum())}}
+        # and on a wide sample: every 25th epoch of the first 64 light curves (compiled
+        # oracle, the reference's route without the polish)
+        nw = min(k, 64)
+        wide = [(kk, e) for kk in range(nw) for e in range(0, n_epochs, 25)]
+        mo = _extended_c_call([kw(kk, e) for kk, e in wide])(0, len(wide), host_cores()[0])
+        gi = np.array([kk * n_epochs + e for kk, e in wide])
+        gv = m64.cpu().numpy()[gi]
+        out["fp64"]["vs_oracle_wide"] = _rel_err_stats(gv, mo)
+        out["fp64"]["vs_oracle_wide"]["sample"] = (f"every 25th epoch of the first {nw} light "
+                                                   "curves, their npts_limb (compiled oracle)")
+        out["fp64"]["vs_oracle_wide"]["largest_differences_vs_truth"] = _truth_judge(
+            gv, mo, [kw(kk, e) for kk, e in wide])
     return out
 
 
