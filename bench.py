@@ -611,9 +611,10 @@ def cpu_rates_omp_items(call, n_items, budget_s, what, unit):
     return out
 
 
-def _extended_c_call(items):
+def _extended_c_call(items, polish=False):
     """call(lo, hi, nthreads) over a list of mag_extended_source keyword sets with one lens
-    count: the compiled oracle on items lo..hi-1"""
+    count: the compiled oracle on items lo..hi-1 (polish: with the canonical root polish
+    the GPU path applies)"""
     from oracle import extended as OE
     names = ("s", "q", "q3", "r3", "psi")
     cols = {k: np.array([it[k] for it in items]) for k in ("w0", "rho", "npts_limb", "u1")}
@@ -623,7 +624,7 @@ def _extended_c_call(items):
     def call(lo, hi, nt):
         m, bad = OE.mag_extended_source_c(
             cols["w0"][lo:hi], cols["rho"][lo:hi], nl, npts_limb=cols["npts_limb"][lo:hi],
-            limb_darkening=True, u1=cols["u1"][lo:hi], npts_ld=100, nthreads=nt,
+            limb_darkening=True, u1=cols["u1"][lo:hi], npts_ld=100, nthreads=nt, polish=polish,
             **{k: v[lo:hi] for k, v in prm.items()})
         return m
     return call
@@ -699,6 +700,8 @@ def bench_extended_binary(steps, warmup, n_near=2048, n_far=2048, cpu_s=0.0):
         out["fp64"]["vs_oracle"] = _rel_err_stats(gv, mo)
         out["fp64"]["vs_oracle"]["sample"] = (f"the first {len(idx)} gate-rejected near-caustic "
                                               "sources, their npts_limb (compiled oracle)")
+        mp_ = _extended_c_call([items[i] for i in idx], polish=True)(0, len(idx), host_cores()[0])
+        out["fp64"]["vs_oracle"]["polished_oracle"] = _rel_err_stats(gv, mp_)
         out["fp64"]["vs_oracle"]["largest_differences_vs_truth"] = _truth_judge(
             gv, mo, [items[i] for i in idx])
     return out
@@ -880,6 +883,9 @@ def bench_triple(steps, warmup, n_sets=1024, n_epochs=1500, budget_s=40.0, cpu_s
         out["fp64"]["vs_oracle_wide"] = _rel_err_stats(gv, mo)
         out["fp64"]["vs_oracle_wide"]["sample"] = (f"every 25th epoch of the first {nw} light "
                                                    "curves, their npts_limb (compiled oracle)")
+        mp_ = _extended_c_call([kw(kk, e) for kk, e in wide], polish=True)(0, len(wide),
+                                                                          host_cores()[0])
+        out["fp64"]["vs_oracle_wide"]["polished_oracle"] = _rel_err_stats(gv, mp_)
         out["fp64"]["vs_oracle_wide"]["largest_differences_vs_truth"] = _truth_judge(
             gv, mo, [kw(kk, e) for kk, e in wide])
     return out

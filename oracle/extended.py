@@ -429,11 +429,12 @@ def mag_extended_source(w0, rho, nlenses=2, npts_limb=150, limb_darkening=False,
 
 
 def mag_extended_source_c(w0, rho, nlenses=2, npts_limb=150, limb_darkening=False, u1=0.0,
-                          npts_ld=100, nthreads=1, **params):
+                          npts_ld=100, nthreads=1, polish=False, **params):
     """mag_extended_source in compiled C++ (extended_oracle.cpp: ora_mag_extended_source)
     with OpenMP over sources: the same steps, for the host CPU baseline.  w0, rho,
-    npts_limb, u1 and the lens parameters broadcast against each other; returns the
-    magnifications and the number of sources with an unconverged limb solve."""
+    npts_limb, u1 and the lens parameters broadcast against each other; polish as in
+    images_of_source_limb.  Returns the magnifications and the number of sources with an
+    unconverged limb solve."""
     import ctypes
 
     from . import ea
@@ -456,11 +457,11 @@ def mag_extended_source_c(w0, rho, nlenses=2, npts_limb=150, limb_darkening=Fals
     f.restype = ctypes.c_int64
     vp, i32 = ctypes.c_void_p, ctypes.c_int
     f.argtypes = [i32, ctypes.c_int64, vp, vp, vp, ctypes.c_int64, vp, i32, vp, vp, vp, i32, vp,
-                  vp, i32, vp, i32]
+                  vp, i32, i32, vp, i32]
     bad = f(nlenses, w.size, w.ctypes.data, r.ctypes.data, prm.ctypes.data, prm.shape[1],
             npts.ctypes.data, int(bool(limb_darkening)), u.ctypes.data, x1.ctypes.data,
-            w1.ctypes.data, n1, x2.ctypes.data, w2.ctypes.data, npts_ld - n1, out.ctypes.data,
-            nthreads)
+            w1.ctypes.data, n1, x2.ctypes.data, w2.ctypes.data, npts_ld - n1, int(bool(polish)),
+            out.ctypes.data, nthreads)
     if bad < 0:
         raise ValueError("nlenses must be 2 or 3")
     return out.reshape(shape), int(bad)

@@ -37,10 +37,12 @@ struct cplx {
 extern "C" int ora_ehrlich_aberth(int deg, int itmax, int compensated, int custom_init,
                                   int init_mode, const cplx *poly, const cplx *roots_init,
                                   cplx *roots, int64_t *stats);
+extern "C" void ora_polish(int deg, const cplx *p, cplx *z);
 
 // ---- lens ----------------------------------------------------------------------------
 struct Lens {
     int nl = 2, n = 5;       // lenses, images (polynomial degree)
+    bool polish = false;     // end every limb solve with ora_polish (canonical roots)
     double a = 0, x_cm = 0;
     dc r[3];
     double e[3];
@@ -164,6 +166,7 @@ bool solve_point(dc w, const Lens &L, dc *z, double *mask, double *par) {
     cplx poly[11], roots[10];
     std::memcpy(poly, asc, sizeof(cplx) * (L.n + 1));
     const bool ok = ora_ehrlich_aberth(L.n, 2500, 0, 0, 0, poly, nullptr, roots, nullptr) != 0;
+    if (L.polish) ora_polish(L.n, poly, roots);
     for (int i = 0; i < L.n; ++i) {
         z[i] = dc(roots[i].re, roots[i].im);
         mask[i] = std::abs(lens_eq(z[i], L) - w) < 1e-6 ? 1.0 : 0.0;
@@ -693,8 +696,9 @@ double integrate_ld(const std::vector<dc> &z, int tidx, const LD &d) {
 
 double mag_one(dc w0u, double rho, int nl, const double *prm, int npts, bool ld, double u1,
                const double *x1, const double *w1, int n1, const double *x2, const double *w2,
-               int n2, bool &ok) {
-    const Lens L = make_lens(nl, prm);
+               int n2, bool polish, bool &ok) {
+    Lens L = make_lens(nl, prm);
+    L.polish = polish;
     const dc w0 = w0u + L.x_cm;
     Limb lb;
     ok = images_of_source_limb(w0, rho, npts, L, lb);
@@ -757,14 +761,16 @@ double mag_one(dc w0u, double rho, int nl, const double *prm, int npts, bool ld,
 
 // mag[i] for sources (w0 interleaved complex in the reference's frame, rho, parameter rows
 // (s, q[, q3, r3, psi]) at params + i * pstride, npts_limb per source, u1 per source);
-// limb darkening with the given Gauss-Legendre nodes of the two half-lines.  nthreads <= 0:
-// the OpenMP default.  Returns the number of sources with an unconverged solve.
+// limb darkening with the given Gauss-Legendre nodes of the two half-lines.  polish != 0
+// ends every limb solve with the canonical compensated polish (ora_polish; no reference
+// counterpart), as oracle/extended.py does with polish=True.  nthreads <= 0: the OpenMP
+// default.  Returns the number of sources with an unconverged solve.
 extern "C" int64_t ora_mag_extended_source(int nlenses, int64_t S, const double *w0,
                                            const double *rho, const double *params,
                                            int64_t pstride, const int32_t *npts, int ld,
                                            const double *u1, const double *x1, const double *w1,
                                            int n1, const double *x2, const double *w2, int n2,
-                                           double *mag, int nthreads) {
+                                           int polish, double *mag, int nthreads) {
     if (nlenses != 2 && nlenses != 3) return -1;
     int64_t bad = 0;
 #ifdef _OPENMP
@@ -774,7 +780,8 @@ extern "C" int64_t ora_mag_extended_source(int nlenses, int64_t S, const double 
     for (int64_t i = 0; i < S; ++i) {
         bool ok = true;
         mag[i] = mag_one(dc(w0[2 * i], w0[2 * i + 1]), rho[i], nlenses, params + i * pstride,
-                         npts[i], ld != 0, ld ? u1[i] : 0.0, x1, w1, n1, x2, w2, n2, ok);
+                         npts[i], ld != 0, ld ? u1[i] : 0.0, x1, w1, n1, x2, w2, n2,
+                         polish != 0, ok);
         bad += ok ? 0 : 1;
     }
     return bad;

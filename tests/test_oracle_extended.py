@@ -208,6 +208,13 @@ def test_compiled_extended_source_matches_numpy_oracle():
         n_open += not parts["all_closed"]
         assert abs(float(b) - a) <= 1e-10 * a, (nl, w0, rho, ld, npts, a, float(b))
     assert n_cmp >= 20 and n_open >= 16, (n_cmp, n_open)
+    # the optional canonical root polish, the same in both (observed 1.4e-13)
+    for nl, w0, rho, ld, npts, prm in cases[:4] + cases[12:16]:
+        a = E.mag_extended_source(w0, rho, nl, npts_limb=npts, limb_darkening=ld, u1=0.4,
+                                  polish=True, **prm)
+        b, _ = E.mag_extended_source_c(w0, rho, nl, npts_limb=npts, limb_darkening=ld, u1=0.4,
+                                       polish=True, **prm)
+        assert abs(float(b) - a) <= 1e-10 * a, (nl, w0, rho, ld, npts, a, float(b))
     for nl in (2, 3):
         sub = [c for c in cases if c[0] == nl and c[3]]
         keys = ("s", "q") if nl == 2 else ("s", "q", "q3", "r3", "psi")
