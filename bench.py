@@ -702,13 +702,13 @@ def bench_extended_binary(steps, warmup, n_near=2048, n_far=2048, cpu_s=0.0):
                                               "sources, their npts_limb (compiled oracle)")
         mp_ = _extended_c_call([items[i] for i in idx], polish=True)(0, len(idx), host_cores()[0])
         out["fp64"]["vs_oracle"]["polished_oracle"] = _rel_err_stats(gv, mp_)
-        out["fp64"]["vs_oracle"]["largest_differences_vs_truth"] = _truth_judge(
+        out["fp64"]["vs_oracle"]["largest_differences_vs_truth"] = _closer_to_truth(
             gv, mo, [items[i] for i in idx])
     return out
 
 
-def _truth_judge(gpu, ref, kws, top=20):
-    """The largest GPU-oracle differences judged against the source-plane quadrature of
+def _closer_to_truth(gpu, ref, kws, top=20):
+    """The largest GPU-oracle differences compared with the source-plane quadrature of
     the same disk (oracle/extended.py mag_disk_quadrature at 200 nodes, 100 for the
     resolution gap): which of the two lies closer to it."""
     import multiprocessing as mp
@@ -886,7 +886,7 @@ def bench_triple(steps, warmup, n_sets=1024, n_epochs=1500, budget_s=40.0, cpu_s
         mp_ = _extended_c_call([kw(kk, e) for kk, e in wide], polish=True)(0, len(wide),
                                                                           host_cores()[0])
         out["fp64"]["vs_oracle_wide"]["polished_oracle"] = _rel_err_stats(gv, mp_)
-        out["fp64"]["vs_oracle_wide"]["largest_differences_vs_truth"] = _truth_judge(
+        out["fp64"]["vs_oracle_wide"]["largest_differences_vs_truth"] = _closer_to_truth(
             gv, mo, [kw(kk, e) for kk, e in wide])
     return out
 
