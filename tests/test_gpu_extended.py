@@ -365,3 +365,48 @@ def test_mag_autograd_matches_fused(gpu):
     np.testing.assert_allclose(st.grad.item(), go[:, 4].sum().item(), rtol=1e-12)
     np.testing.assert_allclose(rt.grad.item(), go[:, 2].sum().item(), rtol=1e-12)
     assert use.any() and (~use).any()
+
+
+@pytest.mark.parametrize("nlenses", [2, 3])
+def test_workspace_contents_do_not_change_results(gpu, nlenses):
+    """Every workspace read is of memory the same launch sequence wrote before: a
+    workspace holding the 64-bit value 1 in every word gives the bits of a zero-filled
+    one (a stale index would read 0 or 1 and a stale double a tiny finite number, so the
+    check addresses nothing outside the workspace), forward and reverse mode, limb
+    darkened, ragged, with a skipped source."""
+    from caustics_amd import _lib
+    from caustics_amd.extended_source import extended_workspace_bytes
+
+    prm_d = dict(s=0.9, q=0.2) if nlenses == 2 else dict(s=1.2, q=0.3, q3=0.01, r3=0.8, psi=1.0)
+    names = list(prm_d)
+    rho = 1e-2
+    w = caustic_points(rho, nlenses=nlenses, npts=6, **prm_d)
+    S = len(w)
+    g = np.random.default_rng(5)
+    npts = g.choice([128, 256, 512], S).astype(np.int32)
+    npts[1] = 0   # skipped
+    nm = int(npts.max())
+    wt = torch.as_tensor(w, dtype=torch.complex128, device="cuda")
+    prm = torch.as_tensor(np.tile([prm_d[k] for k in names], (S, 1)), device="cuda").contiguous()
+    rho_t = torch.full((S,), rho, dtype=torch.float64, device="cuda")
+    u1 = torch.full((S,), 0.4, dtype=torch.float64, device="cuda")
+    npts_t = torch.as_tensor(npts, device="cuda")
+    gm = torch.as_tensor(g.normal(size=S), device="cuda")
+    nb = extended_workspace_bytes(nlenses, S, nm)
+    L = _lib.lib()
+    out = []
+    for fill in (0, 1):
+        ws = torch.full(((nb + 7) // 8,), fill, dtype=torch.int64, device="cuda")
+        mag = torch.empty(S, dtype=torch.float64, device="cuda")
+        st = torch.empty(S, dtype=torch.uint8, device="cuda")
+        go = torch.empty((S, 4 + len(names)), dtype=torch.float64, device="cuda")
+        _lib.check(L.caus_mag_extended_source_work(
+            _lib.stream_ptr(), nlenses, _lib.ptr(wt), S, _lib.ptr(rho_t), _lib.ptr(prm),
+            len(names), _lib.ptr(npts_t), nm, 1, _lib.ptr(u1), 1, 50, _lib.ptr(gm),
+            _lib.ptr(mag), _lib.ptr(go), _lib.ptr(st), _lib.ptr(ws), ws.numel() * 8, 2500,
+            _lib.MODE_FP64, None), "caus_mag_extended_source_work")
+        torch.cuda.synchronize()
+        out.append((mag, go, st))
+    for a, b in zip(*out):
+        assert torch.equal(a, b)
+    assert out[0][0][1].item() == 0 and bool(torch.isfinite(out[0][0]).all())
